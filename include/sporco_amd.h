@@ -1,0 +1,144 @@
+/*
+ * sporco_amd.h - C ABI of the MI355X (gfx950) convolutional BPDN ADMM solvers.
+ *
+ * Drop-in replacement of the C interface that the reference's Cython layer
+ * binds (sporco_cuda/cbpdn.pyx:11-53, sporco_cuda/util.pyx:3-7).  Every
+ * function returns a status code instead of calling exit() (the reference's
+ * error convention, sporco_cuda/src/error_check.cuh:16-21 and
+ * sporco_cuda/src/utils.cu:106-124), never changes the caller's current
+ * device, and never keeps a pointer to caller memory after it returns.
+ *
+ * Array layouts are the reference's C order: D (M, Kh, Kw), S (R, C),
+ * Y (M, R, C), L1 weights (1), (M) or (M, R, C), mask (R, C) int32.
+ */
+#ifndef SPORCO_AMD_H
+#define SPORCO_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes (the reference has none: every failure exits the process). */
+typedef enum scc_status {
+  SCC_OK = 0,
+  SCC_ERR_INVALID = 1,     /* bad sizes, options or pointers */
+  SCC_ERR_NOMEM = 2,       /* device allocation failed */
+  SCC_ERR_NODEVICE = 3,    /* no HIP device, or device index out of range */
+  SCC_ERR_HIP = 4,         /* a HIP runtime call or kernel launch failed */
+  SCC_ERR_UNSUPPORTED = 5  /* size outside what the kernels implement */
+} scc_status;
+
+/*
+ * Algorithm options and problem sizes.  Same fields, names and meaning as
+ * the reference's struct AlgOpt (sporco_cuda/src/algopt.h:7-48), so that the
+ * Cython marshalling (cbpdn.pyx:57-79, 158-201, 297-349) carries over field
+ * by field.  Differences: weight pointers are const; RhoRsdlTarget <= 0
+ * means "derive from lambda" (sporco's None) instead of being overwritten
+ * unconditionally (cbpdn.cu:159-162).
+ */
+typedef struct scc_algopt {
+  int MaxMainIter;
+  float rho;
+  float RhoRsdlRatio;
+  float RhoScaling;
+  float RhoRsdlTarget;
+  float RelaxParam;
+  float AbsStopTol;
+  float RelStopTol;
+  const float *L1Weight;
+  int L1_WEIGHT_M_SIZE;
+  int L1_WEIGHT_ROW_SIZE;
+  int L1_WEIGHT_COL_SIZE;
+  const int *Weight; /* AMS mask (R, C), used when nWeight > 1 */
+  int nWeight;
+  int Verbose;
+  int NonNegCoef;
+  int NoBndryCross;
+  int AuxVarObj;
+  int HighMemSolve;
+  int AutoRho;
+  int AutoRhoScaling;
+  int AutoRhoPeriod;
+  int StdResiduals;
+  int IMG_ROW_SIZE;
+  int IMG_COL_SIZE;
+  int DICT_M_SIZE;
+  int DICT_ROW_SIZE;
+  int DICT_COL_SIZE;
+  const float *GrdWeight;
+  int WEIGHT_SIZE;
+  int device;
+} scc_algopt;
+
+/* Outcome of the last solve on the calling thread. */
+typedef struct scc_info {
+  int iterations;   /* ADMM iterations run */
+  float rho;        /* final penalty parameter */
+  float r, s;       /* final primal / dual residuals (normalised unless
+                       StdResiduals) */
+  float epri, edua; /* final stopping thresholds */
+  double setup_ms;  /* device time of setup (spectra of D and S) */
+  double solve_ms;  /* device time of the ADMM iterations */
+} scc_info;
+
+/* Replaces clear_opts (sporco_cuda/src/utils.cu:66-98): sets the -1
+ * sentinels.  Unlike the reference it allocates nothing. */
+void scc_clear_opts(scc_algopt *opt);
+
+/* Replaces default_opts (sporco_cuda/src/utils.cu:12-64): fills sentinels
+ * with defaults (RelaxParam 1.8, RelStopTol 1e-4, MaxMainIter 250, ...). */
+void scc_default_opts(scc_algopt *opt);
+
+/* Replaces cuda_wrapper_CBPDN (sporco_cuda/src/cbpdn.h:10,
+ * sporco_cuda/src/cbpdn.cu:86-616): plain CBPDN, and CBPDN with an AMS
+ * mask when opt->nWeight > 1 (filter 0 of D is then the impulse).
+ * Host pointers; Y (M, R, C) is overwritten.  opt is read-only. */
+int scc_cbpdn(const float *D, const float *S, float lambda,
+              const scc_algopt *opt, float *Y);
+
+/* Replaces cuda_wrapper_CBPDN_GR (sporco_cuda/src/cbpdn_grd.h:10-11,
+ * sporco_cuda/src/cbpdn_grd.cu:97-658): gradient-regularised CBPDN, masked
+ * when opt->nWeight > 1. */
+int scc_cbpdn_grd(const float *D, const float *S, float lambda, float mu,
+                  const scc_algopt *opt, float *Y);
+
+/* Device-pointer forms of the two solvers for callers whose data already
+ * lives in HBM.  All pointers in opt (L1Weight, Weight, GrdWeight) and the
+ * array arguments are device pointers; work is queued on `stream` (a
+ * hipStream_t, NULL = default stream) and the call returns after the
+ * solve has completed.  No counterpart in the reference, whose C entry
+ * points take host arrays only (cbpdn.cu:249-260, 575). */
+int scc_cbpdn_device(const float *D, const float *S, float lambda, float mu,
+                     int grad, const scc_algopt *opt, float *Y, void *stream);
+
+/* Info and error text of the last call on this thread (no reference
+ * counterpart: the reference prints and exits). */
+int scc_last_info(scc_info *info);
+const char *scc_last_error(void);
+
+/* Per-iteration history (r, s, rho) of the last solve on this thread, when
+ * enabled with scc_set_history(1).  Writes up to cap triples; returns the
+ * number of iterations recorded.  No reference counterpart. */
+void scc_set_history(int enabled);
+int scc_last_history(float *rsr, int cap);
+
+/* Device queries: replace get_device_count, get_current_device,
+ * set_current_device, get_memory_info, get_device_name
+ * (sporco_cuda/src/utils.cu:146-175, bound at util.pyx:3-7). */
+int scc_get_device_count(void);
+int scc_get_current_device(int *dev);
+int scc_set_current_device(int dev);
+int scc_get_memory_info(size_t *free_bytes, size_t *total_bytes);
+const char *scc_get_device_name(int dev);
+
+/* Release cached plans and device buffers (no reference counterpart; the
+ * reference allocates and frees per call, cbpdn.cu:186-215, 583-615). */
+void scc_release_cache(void);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* SPORCO_AMD_H */

@@ -1,0 +1,690 @@
+// Host driver and C ABI of the CBPDN ADMM solvers (include/sporco_amd.h).
+//
+// Replaces the reference host drivers cuda_wrapper_CBPDN
+// (sporco_cuda/src/cbpdn.cu:86-616) and cuda_wrapper_CBPDN_GR
+// (sporco_cuda/src/cbpdn_grd.cu:97-658).  Differences by design:
+//  * a workspace (device buffers, FFT plans, twiddle tables) is cached per
+//    device and shape instead of 14 allocations and 3 FFT plans per call;
+//  * the iteration never waits for the host: residual norms, the stopping
+//    test and AutoRho run on the device (k_reduce), and every kernel of a
+//    later iteration returns at once after the stopping test has passed; the
+//    host only polls the device flag every few iterations to stop queueing;
+//  * errors are returned as status codes, the caller's device is restored.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/sporco_amd.h"
+#include "scc_internal.h"
+#include "scc_pow2.h"
+
+namespace scc {
+namespace {
+
+thread_local std::string g_err;
+thread_local scc_info g_info;
+thread_local bool g_hist_on = false;
+thread_local std::vector<float> g_hist;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define SCC_HIP(call)                                                  \
+  do {                                                                 \
+    hipError_t e_ = (call);                                            \
+    if (e_ != hipSuccess) {                                            \
+      return fail(e_ == hipErrorOutOfMemory ? SCC_ERR_NOMEM           \
+                                            : SCC_ERR_HIP,            \
+                  std::string(#call) + ": " + hipGetErrorString(e_)); \
+    }                                                                  \
+  } while (0)
+
+// Radix decomposition for the generic Stockham kernels: 4s, then 2s, then
+// odd factors in increasing order.
+bool make_plan(int L, int tw_stride, FftPlan* pl) {
+  pl->L = L;
+  pl->tw_stride = tw_stride;
+  pl->nstages = 0;
+  int n = L;
+  auto push = [&](int r) {
+    if (pl->nstages >= kMaxStages) return false;
+    pl->radix[pl->nstages++] = r;
+    return true;
+  };
+  while (n % 4 == 0) {
+    if (!push(4)) return false;
+    n /= 4;
+  }
+  while (n % 2 == 0) {
+    if (!push(2)) return false;
+    n /= 2;
+  }
+  for (int p = 3; n > 1; p += 2) {
+    while (n % p == 0) {
+      if (!push(p)) return false;
+      n /= p;
+    }
+  }
+  return true;
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+int ensure(DevBuf& b, size_t bytes) {
+  if (b.bytes >= bytes && b.p) return SCC_OK;
+  if (b.p) {
+    (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  if (bytes == 0) return SCC_OK;
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess)
+    return fail(SCC_ERR_NOMEM, std::string("hipMalloc of ") +
+                                   std::to_string(bytes) + " bytes: " +
+                                   hipGetErrorString(e));
+  b.bytes = bytes;
+  return SCC_OK;
+}
+
+// Everything that depends only on (device, R, C, M): buffers, plans, tables.
+struct Workspace {
+  int device = -1;
+  Geom g{};
+  FftPlan rowp{}, colp{};
+  bool pow2 = false;
+  Pow2Config p2{};
+  DevBuf Y, U, F, Df, Sf, ghg, partials, ctrl, hist, twR, twC;
+  DevBuf sD, sS, sL1, sGW, sMask;  // staging for the host-pointer entry
+  bool ghg_ready = false;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  Ctrl* hctrl = nullptr;  // pinned host copy of the control block
+  ~Workspace() {
+    if (hctrl) (void)hipHostFree(hctrl);
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+std::mutex g_mu;
+std::map<std::tuple<int, int, int, int>, std::unique_ptr<Workspace>> g_ws;
+
+int make_geom(int R, int C, int M, Geom* g, bool* pow2, Pow2Config* p2) {
+  g->R = R;
+  g->C = C;
+  g->M = M;
+  g->H = C / 2 + 1;
+  g->even = (C % 2 == 0) ? 1 : 0;
+  g->Lr = g->even ? C / 2 : C;
+  g->N = (long long)R * C;
+  g->Hc = (long long)R * g->H;
+  *pow2 = pow2_select(R, C, M, p2);
+  int TC;
+  if (*pow2) {
+    TC = p2->TC;
+  } else {
+    int cap = 65536 / (2 * (R + 1) * 8);
+    if (cap < 1) cap = 1;
+    cap = std::min(cap, g->H);
+    TC = 1;
+    while (TC * 2 <= cap) TC *= 2;
+  }
+  g->TC = TC;
+  g->P = (g->H + TC - 1) / TC;
+  g->wlast = g->H - (g->P - 1) * TC;
+  return SCC_OK;
+}
+
+int get_workspace(int device, int R, int C, int M, Workspace** out) {
+  auto key = std::make_tuple(device, R, C, M);
+  auto it = g_ws.find(key);
+  if (it != g_ws.end()) {
+    *out = it->second.get();
+    return SCC_OK;
+  }
+  // Keep at most one workspace per device: shapes of consecutive calls
+  // usually repeat, and large shapes hold gigabytes.
+  for (auto i = g_ws.begin(); i != g_ws.end();) {
+    if (std::get<0>(i->first) == device)
+      i = g_ws.erase(i);
+    else
+      ++i;
+  }
+  auto ws = std::make_unique<Workspace>();
+  ws->device = device;
+  make_geom(R, C, M, &ws->g, &ws->pow2, &ws->p2);
+  const Geom& g = ws->g;
+  if (!make_plan(g.Lr, g.even ? 2 : 1, &ws->rowp) ||
+      !make_plan(R, 1, &ws->colp))
+    return fail(SCC_ERR_UNSUPPORTED, "FFT length has too many factors");
+  const size_t nreal = (size_t)M * g.N;
+  const size_t ncplx = (size_t)M * g.Hc;
+  int rc;
+  if ((rc = ensure(ws->Y, nreal * 4)) || (rc = ensure(ws->U, nreal * 4)) ||
+      (rc = ensure(ws->F, ncplx * 8)) || (rc = ensure(ws->Df, ncplx * 8)) ||
+      (rc = ensure(ws->Sf, (size_t)g.Hc * 8)) ||
+      (rc = ensure(ws->ctrl, sizeof(Ctrl))) ||
+      (rc = ensure(ws->twR, (size_t)R * 8)) ||
+      (rc = ensure(ws->twC, (size_t)C * 8)))
+    return rc;
+  const int nb = ws->pow2 ? pow2_row_blocks(g, ws->p2) : row_blocks(g);
+  if ((rc = ensure(ws->partials, (size_t)5 * nb * 4))) return rc;
+  SCC_HIP(hipStreamCreateWithFlags(&ws->stream, hipStreamNonBlocking));
+  for (auto& e : ws->ev) SCC_HIP(hipEventCreate(&e));
+  SCC_HIP(hipHostMalloc((void**)&ws->hctrl, sizeof(Ctrl)));
+  // Twiddle tables W_L^t = exp(-2 pi i t / L), computed in double.
+  std::vector<float2> tw;
+  for (int pass = 0; pass < 2; ++pass) {
+    const int L = pass == 0 ? R : C;
+    tw.resize(L);
+    for (int t = 0; t < L; ++t) {
+      const double a = -2.0 * M_PI * (double)t / (double)L;
+      tw[t] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    SCC_HIP(hipMemcpy(pass == 0 ? ws->twR.p : ws->twC.p, tw.data(),
+                      (size_t)L * 8, hipMemcpyHostToDevice));
+  }
+  *out = ws.get();
+  g_ws[key] = std::move(ws);
+  return SCC_OK;
+}
+
+int ensure_ghg(Workspace* ws) {
+  if (ws->ghg_ready) return SCC_OK;
+  const Geom& g = ws->g;
+  int rc = ensure(ws->ghg, (size_t)g.Hc * 4);
+  if (rc) return rc;
+  // |F[-1 1]|^2 along rows plus along columns, the closed form of the
+  // spectra the reference computes by FFT (cbpdn_grd.cu:295-314).
+  std::vector<float> h((size_t)g.Hc);
+  for (int r = 0; r < g.R; ++r)
+    for (int c = 0; c < g.H; ++c) {
+      const double v = 4.0 - 2.0 * std::cos(2.0 * M_PI * r / g.R) -
+                       2.0 * std::cos(2.0 * M_PI * c / g.C);
+      h[panel_off(g, r, c)] = (float)v;
+    }
+  SCC_HIP(hipMemcpy(ws->ghg.p, h.data(), h.size() * 4,
+                    hipMemcpyHostToDevice));
+  ws->ghg_ready = true;
+  return SCC_OK;
+}
+
+struct Resolved {
+  scc_algopt o;
+  int l1_mode;   // 0 scalar, 1 per map, 2 full
+  size_t n_l1;
+  int gw_mode;   // 0 scalar, 1 per map
+  bool mask;
+  float xi;
+};
+
+// Apply defaults to a copy of the options and check them before any HIP
+// call, so that invalid input fails the same way with or without a GPU.
+int resolve(const scc_algopt* in, float lambda, int grad, Resolved* rs) {
+  if (in == nullptr) return fail(SCC_ERR_INVALID, "opt is NULL");
+  rs->o = *in;
+  scc_default_opts(&rs->o);
+  const scc_algopt& o = rs->o;
+  const int R = o.IMG_ROW_SIZE, C = o.IMG_COL_SIZE, M = o.DICT_M_SIZE;
+  if (R < 1 || C < 1 || M < 1 || o.DICT_ROW_SIZE < 1 || o.DICT_COL_SIZE < 1)
+    return fail(SCC_ERR_INVALID, "image and dictionary sizes must be >= 1");
+  if (o.DICT_ROW_SIZE > R || o.DICT_COL_SIZE > C)
+    return fail(SCC_ERR_INVALID, "filters are larger than the image");
+  if ((long long)R * C > (1LL << 30) || (long long)M * R * C > (1LL << 36))
+    return fail(SCC_ERR_UNSUPPORTED, "problem too large");
+  // LDS bounds of the generic kernels: 2 (R+1) and 2 (Lr+1) complex values
+  // of one column / row must fit in 64 KB with the reduction scratch.
+  if (R > 4095 || (C % 2 == 0 ? C > 7678 : C > 3839))
+    return fail(SCC_ERR_UNSUPPORTED, "image side too large");
+  if (o.MaxMainIter < 0)
+    return fail(SCC_ERR_INVALID, "MaxMainIter must be >= 0");
+  if (!(o.rho > 0.f) || !std::isfinite(o.rho))
+    return fail(SCC_ERR_INVALID, "rho must be positive and finite");
+  if (!std::isfinite(lambda))
+    return fail(SCC_ERR_INVALID, "lambda must be finite");
+  if (o.AutoRho == 1 && o.AutoRhoPeriod < 1)
+    return fail(SCC_ERR_INVALID, "AutoRhoPeriod must be >= 1");
+  if (o.L1Weight == nullptr)
+    return fail(SCC_ERR_INVALID, "L1Weight is NULL");
+  const long long nl1 = (long long)std::max(o.L1_WEIGHT_M_SIZE, 0) *
+                        std::max(o.L1_WEIGHT_ROW_SIZE, 0) *
+                        std::max(o.L1_WEIGHT_COL_SIZE, 0);
+  // Same dispatch order as cbpdn.cu:368-389.
+  if (nl1 == M) {
+    rs->l1_mode = 1;
+  } else if (nl1 == 1) {
+    rs->l1_mode = 0;
+  } else if (nl1 == (long long)M * R * C) {
+    rs->l1_mode = 2;
+  } else {
+    return fail(SCC_ERR_INVALID,
+                "L1Weight must hold 1, M or M*R*C entries");
+  }
+  rs->n_l1 = (size_t)nl1;
+  rs->gw_mode = 0;
+  if (grad) {
+    if (o.GrdWeight == nullptr)
+      return fail(SCC_ERR_INVALID, "GrdWeight is NULL");
+    if (o.WEIGHT_SIZE == M && M > 1)
+      rs->gw_mode = 1;
+    else if (o.WEIGHT_SIZE != 1 && o.WEIGHT_SIZE != M)
+      return fail(SCC_ERR_INVALID, "GradWeight must hold 1 or M entries");
+  }
+  rs->mask = o.nWeight > 1;
+  if (rs->mask) {
+    if (o.Weight == nullptr)
+      return fail(SCC_ERR_INVALID, "Weight (mask) is NULL");
+    if ((long long)o.nWeight != (long long)R * C)
+      return fail(SCC_ERR_INVALID, "mask must hold R*C entries");
+  }
+  float xi = o.RhoRsdlTarget;
+  if (!(xi > 0.f))
+    xi = o.StdResiduals == 1
+             ? 1.f
+             : (float)(1.0 + std::pow(18.3, std::log10((double)lambda) + 1.0));
+  rs->xi = xi;
+  return SCC_OK;
+}
+
+int check_device(int dev) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0)
+    return fail(SCC_ERR_NODEVICE, "no HIP device available");
+  if (dev < 0 || dev >= n)
+    return fail(SCC_ERR_NODEVICE, "device index out of range");
+  return SCC_OK;
+}
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+// The ADMM solve on device-resident inputs.  D (M,Kh,Kw), S (R,C), weights
+// and mask are device pointers; Y (M,R,C) receives the result on the device.
+int solve_core(Workspace* ws, const Resolved& rs, const float* D,
+               const float* S, float lambda, float mu, int grad,
+               const float* l1w, const float* gw, const int* mask, float* Yout,
+               hipStream_t st) {
+  const scc_algopt& o = rs.o;
+  const Geom& g = ws->g;
+  const int M = g.M;
+  Params p{};
+  p.lambda = lambda;
+  p.mu = mu;
+  p.alpha = o.RelaxParam;
+  p.abs_tol = o.AbsStopTol;
+  p.rel_tol = o.RelStopTol;
+  p.tau = o.RhoScaling;
+  p.rsdl_ratio = o.RhoRsdlRatio;
+  p.xi = rs.xi;
+  p.sqrt_nc = std::sqrt((double)M * (double)g.N);
+  p.std_residuals = o.StdResiduals == 1;
+  p.autorho = o.AutoRho == 1;
+  p.autoscaling = o.AutoRhoScaling == 1;
+  p.period = std::max(o.AutoRhoPeriod, 1);
+  p.nonneg = o.NonNegCoef == 1;
+  p.nobndry = o.NoBndryCross == 1;
+  p.Kh = o.DICT_ROW_SIZE;
+  p.Kw = o.DICT_COL_SIZE;
+  p.m0 = rs.mask ? 1 : 0;
+  p.l1_mode = rs.l1_mode;
+  p.grad = grad ? 1 : 0;
+  p.gw_mode = rs.gw_mode;
+  p.track = g_hist_on ? 1 : 0;
+  p.l1_scalar = 1.f;
+  p.gw_scalar = 1.f;
+  if (rs.l1_mode == 0)
+    SCC_HIP(hipMemcpyAsync(&p.l1_scalar, l1w, 4, hipMemcpyDeviceToHost, st));
+  if (grad && rs.gw_mode == 0)
+    SCC_HIP(hipMemcpyAsync(&p.gw_scalar, gw, 4, hipMemcpyDeviceToHost, st));
+  SCC_HIP(hipStreamSynchronize(st));
+  int rc;
+  if (grad && (rc = ensure_ghg(ws))) return rc;
+  const int maxit = o.MaxMainIter;
+  if (p.track && (rc = ensure(ws->hist, (size_t)3 * std::max(maxit, 1) * 4)))
+    return rc;
+
+  Buffers b{};
+  b.Y = (float*)ws->Y.p;
+  b.U = (float*)ws->U.p;
+  b.F = (float2*)ws->F.p;
+  b.Df = (float2*)ws->Df.p;
+  b.Sf = (float2*)ws->Sf.p;
+  b.ghg = (float*)ws->ghg.p;
+  b.l1w = l1w;
+  b.gw = gw;
+  b.mask = mask;
+  b.partials = (float*)ws->partials.p;
+  b.ctrl = (Ctrl*)ws->ctrl.p;
+  b.hist = p.track ? (float*)ws->hist.p : nullptr;
+  b.twR = (float2*)ws->twR.p;
+  b.twC = (float2*)ws->twC.p;
+
+  const size_t nreal = (size_t)M * g.N;
+  const size_t ncplx = (size_t)M * g.Hc;
+  SCC_HIP(hipEventRecord(ws->ev[0], st));
+  // Setup (cbpdn.cu:276-298): spectra of the padded filters and the image.
+  SCC_HIP(launch_pad(D, b.Y, M, o.DICT_ROW_SIZE, o.DICT_COL_SIZE, g.R, g.C,
+                     st));
+  if (ws->pow2) {
+    SCC_HIP(pow2_row_fwd(g, ws->p2, M, kFwdReal, b.Y, nullptr, nullptr, b.Df,
+                         b.twC, nullptr, st));
+    SCC_HIP(pow2_col(g, ws->p2, M, b.Df, b.twR, 0, nullptr, st));
+    SCC_HIP(pow2_row_fwd(g, ws->p2, 1, kFwdReal, S, nullptr, nullptr, b.Sf,
+                         b.twC, nullptr, st));
+    SCC_HIP(pow2_col(g, ws->p2, 1, b.Sf, b.twR, 0, nullptr, st));
+  } else {
+    SCC_HIP(launch_row_fwd(g, M, ws->rowp, kFwdReal, b.Y, nullptr, nullptr,
+                           b.Df, b.twC, nullptr, st));
+    SCC_HIP(launch_col(g, M, ws->colp, b.Df, b.twR, 0, nullptr, st));
+    SCC_HIP(launch_row_fwd(g, 1, ws->rowp, kFwdReal, S, nullptr, nullptr,
+                           b.Sf, b.twC, nullptr, st));
+    SCC_HIP(launch_col(g, 1, ws->colp, b.Sf, b.twR, 0, nullptr, st));
+  }
+  // Y = U = 0 and A-hat = 0 for the first iteration (cbpdn.cu:321-326).
+  SCC_HIP(hipMemsetAsync(b.Y, 0, nreal * 4, st));
+  SCC_HIP(hipMemsetAsync(b.U, 0, nreal * 4, st));
+  SCC_HIP(hipMemsetAsync(b.F, 0, ncplx * 8, st));
+  Ctrl c0{};
+  c0.rho = o.rho;
+  c0.rsf = 1.f;
+  *ws->hctrl = c0;
+  SCC_HIP(hipMemcpyAsync(b.ctrl, ws->hctrl, sizeof(Ctrl),
+                         hipMemcpyHostToDevice, st));
+  SCC_HIP(hipEventRecord(ws->ev[1], st));
+
+  // A stopping test can pass only with a positive tolerance.
+  const bool can_stop = o.RelStopTol > 0.f || o.AbsStopTol > 0.f;
+  const int poll = o.Verbose == 1 ? 1 : 16;
+  int nblocks = 0;
+  for (int i = 1; i <= maxit; ++i) {
+    const bool rho_step = p.autorho && i > 1 && i % p.period == 0;
+    if (ws->pow2) {
+      SCC_HIP(pow2_col(g, ws->p2, M, b.F, b.twR, 0, b.ctrl, st));
+      SCC_HIP(pow2_solve(g, ws->p2, b, p, st));
+      SCC_HIP(pow2_col(g, ws->p2, M, b.F, b.twR, 1, b.ctrl, st));
+      SCC_HIP(pow2_row(g, ws->p2, b, p, rho_step ? kRowUpdate : kRowFused,
+                       &nblocks, st));
+    } else {
+      SCC_HIP(launch_col(g, M, ws->colp, b.F, b.twR, 0, b.ctrl, st));
+      SCC_HIP(launch_solve(g, b, p, st));
+      SCC_HIP(launch_col(g, M, ws->colp, b.F, b.twR, 1, b.ctrl, st));
+      SCC_HIP(launch_row(g, ws->rowp, b, p, rho_step ? kRowUpdate : kRowFused,
+                         &nblocks, st));
+    }
+    SCC_HIP(launch_reduce(b, p, nblocks, i, maxit, st));
+    if (rho_step) {
+      if (ws->pow2)
+        SCC_HIP(pow2_row_fwd(g, ws->p2, M, kFwdYU, nullptr, b.Y, b.U, b.F,
+                             b.twC, b.ctrl, st));
+      else
+        SCC_HIP(launch_row_fwd(g, M, ws->rowp, kFwdYU, nullptr, b.Y, b.U, b.F,
+                               b.twC, b.ctrl, st));
+    }
+    if ((can_stop || o.Verbose == 1) && (i % poll == 0 || i == maxit)) {
+      SCC_HIP(hipMemcpyAsync(ws->hctrl, b.ctrl, sizeof(Ctrl),
+                             hipMemcpyDeviceToHost, st));
+      SCC_HIP(hipStreamSynchronize(st));
+      if (o.Verbose == 1) {
+        if (i == 1) {
+          std::printf("Itn   r         s         rho\n");
+          std::printf("------------------------------------\n");
+        }
+        if (!ws->hctrl->done || ws->hctrl->iter == i)
+          std::printf("%4d %9.2e %9.2e %9.2e\n", ws->hctrl->iter,
+                      ws->hctrl->r, ws->hctrl->s, ws->hctrl->rho);
+      }
+      if (ws->hctrl->done) break;
+    }
+  }
+  SCC_HIP(hipEventRecord(ws->ev[2], st));
+  if (Yout != nullptr)
+    SCC_HIP(hipMemcpyAsync(Yout, b.Y, nreal * 4, hipMemcpyDeviceToDevice, st));
+  SCC_HIP(hipMemcpyAsync(ws->hctrl, b.ctrl, sizeof(Ctrl),
+                         hipMemcpyDeviceToHost, st));
+  SCC_HIP(hipStreamSynchronize(st));
+  const Ctrl& cf = *ws->hctrl;
+  g_info = scc_info{};
+  g_info.iterations = cf.iter;
+  g_info.rho = cf.rho;
+  g_info.r = cf.r;
+  g_info.s = cf.s;
+  g_info.epri = cf.epri;
+  g_info.edua = cf.edua;
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, ws->ev[0], ws->ev[1]) == hipSuccess)
+    g_info.setup_ms = ms;
+  if (hipEventElapsedTime(&ms, ws->ev[1], ws->ev[2]) == hipSuccess)
+    g_info.solve_ms = ms;
+  if (p.track) {
+    g_hist.resize((size_t)3 * cf.iter);
+    if (cf.iter > 0)
+      SCC_HIP(hipMemcpy(g_hist.data(), b.hist, g_hist.size() * 4,
+                        hipMemcpyDeviceToHost));
+  } else {
+    g_hist.clear();
+  }
+  return SCC_OK;
+}
+
+int solve_host(const float* D, const float* S, float lambda, float mu,
+               int grad, const scc_algopt* opt, float* Y) {
+  if (D == nullptr || S == nullptr || Y == nullptr)
+    return fail(SCC_ERR_INVALID, "NULL array argument");
+  Resolved rs;
+  int rc = resolve(opt, lambda, grad, &rs);
+  if (rc) return rc;
+  const scc_algopt& o = rs.o;
+  if ((rc = check_device(o.device))) return rc;
+  std::lock_guard<std::mutex> lk(g_mu);
+  DeviceGuard dg(o.device);
+  Workspace* ws;
+  const int R = o.IMG_ROW_SIZE, C = o.IMG_COL_SIZE, M = o.DICT_M_SIZE;
+  if ((rc = get_workspace(o.device, R, C, M, &ws))) return rc;
+  const size_t nd = (size_t)M * o.DICT_ROW_SIZE * o.DICT_COL_SIZE;
+  const size_t nl1 = rs.n_l1;
+  const size_t ngw = grad ? (size_t)std::max(o.WEIGHT_SIZE, 1) : 0;
+  if ((rc = ensure(ws->sD, nd * 4)) || (rc = ensure(ws->sS, (size_t)R * C * 4)) ||
+      (rc = ensure(ws->sL1, nl1 * 4)) || (rc = ensure(ws->sGW, ngw * 4)) ||
+      (rc = ensure(ws->sMask, rs.mask ? (size_t)R * C * 4 : 0)))
+    return rc;
+  hipStream_t st = ws->stream;
+  SCC_HIP(hipMemcpyAsync(ws->sD.p, D, nd * 4, hipMemcpyHostToDevice, st));
+  SCC_HIP(hipMemcpyAsync(ws->sS.p, S, (size_t)R * C * 4,
+                         hipMemcpyHostToDevice, st));
+  SCC_HIP(hipMemcpyAsync(ws->sL1.p, o.L1Weight, nl1 * 4,
+                         hipMemcpyHostToDevice, st));
+  if (grad)
+    SCC_HIP(hipMemcpyAsync(ws->sGW.p, o.GrdWeight, ngw * 4,
+                           hipMemcpyHostToDevice, st));
+  if (rs.mask)
+    SCC_HIP(hipMemcpyAsync(ws->sMask.p, o.Weight, (size_t)R * C * 4,
+                           hipMemcpyHostToDevice, st));
+  rc = solve_core(ws, rs, (const float*)ws->sD.p, (const float*)ws->sS.p,
+                  lambda, mu, grad, (const float*)ws->sL1.p,
+                  (const float*)ws->sGW.p, (const int*)ws->sMask.p, nullptr,
+                  st);
+  if (rc) return rc;
+  // Result return (cbpdn.cu:575): Y straight from the workspace.
+  SCC_HIP(hipMemcpy(Y, ws->Y.p, (size_t)M * R * C * 4,
+                    hipMemcpyDeviceToHost));
+  return SCC_OK;
+}
+
+}  // namespace
+}  // namespace scc
+
+using namespace scc;
+
+extern "C" {
+
+void scc_clear_opts(scc_algopt* opt) {
+  if (opt == nullptr) return;
+  std::memset(opt, 0, sizeof(*opt));
+  opt->device = -1;
+  opt->rho = -1;
+  opt->Verbose = -1;
+  opt->AutoRho = -1;
+  opt->nWeight = -1;
+  opt->AuxVarObj = -1;
+  opt->AbsStopTol = -1;
+  opt->NonNegCoef = -1;
+  opt->RelaxParam = -1;
+  opt->RhoScaling = -1;
+  opt->RelStopTol = -1;
+  opt->MaxMainIter = -1;
+  opt->NoBndryCross = -1;
+  opt->HighMemSolve = -1;
+  opt->StdResiduals = -1;
+  opt->RhoRsdlRatio = -1;
+  opt->RhoRsdlTarget = -1;
+  opt->AutoRhoPeriod = -1;
+  opt->AutoRhoScaling = -1;
+  opt->WEIGHT_SIZE = 1;
+  opt->L1_WEIGHT_M_SIZE = 1;
+  opt->L1_WEIGHT_ROW_SIZE = 1;
+  opt->L1_WEIGHT_COL_SIZE = 1;
+}
+
+void scc_default_opts(scc_algopt* opt) {
+  if (opt == nullptr) return;
+  // Defaults of default_opts (sporco_cuda/src/utils.cu:12-64).
+  if (opt->device == -1) opt->device = 0;
+  if (opt->rho == -1) opt->rho = 0;
+  if (opt->Verbose == -1) opt->Verbose = 0;
+  if (opt->AutoRho == -1) opt->AutoRho = 1;
+  if (opt->nWeight == -1) opt->nWeight = 1;
+  if (opt->AuxVarObj == -1) opt->AuxVarObj = 0;
+  if (opt->AbsStopTol == -1) opt->AbsStopTol = 0;
+  if (opt->NonNegCoef == -1) opt->NonNegCoef = 0;
+  if (opt->RelaxParam == -1) opt->RelaxParam = 1.8f;
+  if (opt->RhoScaling == -1) opt->RhoScaling = 100;
+  if (opt->RelStopTol == -1) opt->RelStopTol = 1e-4f;
+  if (opt->MaxMainIter == -1) opt->MaxMainIter = 250;
+  if (opt->NoBndryCross == -1) opt->NoBndryCross = 0;
+  if (opt->HighMemSolve == -1) opt->HighMemSolve = 0;
+  if (opt->StdResiduals == -1) opt->StdResiduals = 0;
+  if (opt->RhoRsdlRatio == -1) opt->RhoRsdlRatio = 1.2f;
+  if (opt->AutoRhoPeriod == -1) opt->AutoRhoPeriod = 1;
+  if (opt->AutoRhoScaling == -1) opt->AutoRhoScaling = 1;
+}
+
+int scc_cbpdn(const float* D, const float* S, float lambda,
+              const scc_algopt* opt, float* Y) {
+  return solve_host(D, S, lambda, 0.f, 0, opt, Y);
+}
+
+int scc_cbpdn_grd(const float* D, const float* S, float lambda, float mu,
+                  const scc_algopt* opt, float* Y) {
+  return solve_host(D, S, lambda, mu, 1, opt, Y);
+}
+
+int scc_cbpdn_device(const float* D, const float* S, float lambda, float mu,
+                     int grad, const scc_algopt* opt, float* Y,
+                     void* stream) {
+  if (D == nullptr || S == nullptr || Y == nullptr)
+    return fail(SCC_ERR_INVALID, "NULL array argument");
+  Resolved rs;
+  int rc = resolve(opt, lambda, grad, &rs);
+  if (rc) return rc;
+  const scc_algopt& o = rs.o;
+  if ((rc = check_device(o.device))) return rc;
+  std::lock_guard<std::mutex> lk(g_mu);
+  DeviceGuard dg(o.device);
+  Workspace* ws;
+  if ((rc = get_workspace(o.device, o.IMG_ROW_SIZE, o.IMG_COL_SIZE,
+                          o.DICT_M_SIZE, &ws)))
+    return rc;
+  hipStream_t st = stream ? (hipStream_t)stream : ws->stream;
+  return solve_core(ws, rs, D, S, lambda, mu, grad, o.L1Weight, o.GrdWeight,
+                    o.Weight, Y, st);
+}
+
+int scc_last_info(scc_info* info) {
+  if (info == nullptr) return SCC_ERR_INVALID;
+  *info = g_info;
+  return SCC_OK;
+}
+
+const char* scc_last_error(void) { return g_err.c_str(); }
+
+void scc_set_history(int enabled) { g_hist_on = enabled != 0; }
+
+int scc_last_history(float* rsr, int cap) {
+  const int n = (int)(g_hist.size() / 3);
+  if (rsr != nullptr && cap > 0)
+    std::memcpy(rsr, g_hist.data(), (size_t)std::min(n, cap) * 3 * 4);
+  return n;
+}
+
+int scc_get_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int scc_get_current_device(int* dev) {
+  if (dev == nullptr) return SCC_ERR_INVALID;
+  hipError_t e = hipGetDevice(dev);
+  if (e != hipSuccess) return fail(SCC_ERR_NODEVICE, hipGetErrorString(e));
+  return SCC_OK;
+}
+
+int scc_set_current_device(int dev) {
+  int rc = check_device(dev);
+  if (rc) return rc;
+  hipError_t e = hipSetDevice(dev);
+  if (e != hipSuccess) return fail(SCC_ERR_HIP, hipGetErrorString(e));
+  return SCC_OK;
+}
+
+int scc_get_memory_info(size_t* free_bytes, size_t* total_bytes) {
+  if (free_bytes == nullptr || total_bytes == nullptr) return SCC_ERR_INVALID;
+  int rc = check_device(0);
+  if (rc) return rc;
+  hipError_t e = hipMemGetInfo(free_bytes, total_bytes);
+  if (e != hipSuccess) return fail(SCC_ERR_HIP, hipGetErrorString(e));
+  return SCC_OK;
+}
+
+const char* scc_get_device_name(int dev) {
+  static thread_local hipDeviceProp_t prop;
+  if (check_device(dev)) return nullptr;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return nullptr;
+  return prop.name;
+}
+
+void scc_release_cache(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_ws.clear();
+}
+
+}  // extern "C"

@@ -1,0 +1,170 @@
+// Device helpers shared by the generic and power-of-two kernels: complex
+// arithmetic, the fused per-coefficient update, block reductions of the
+// residual partial sums and the device-side stopping / AutoRho logic.
+#pragma once
+
+#include "scc_internal.h"
+
+namespace scc {
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) {
+  return make_float2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ float2 csub(float2 a, float2 b) {
+  return make_float2(a.x - b.x, a.y - b.y);
+}
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cconj(float2 a) {
+  return make_float2(a.x, -a.y);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) {
+  return make_float2(a.x * s, a.y * s);
+}
+// -i * dir * a
+__device__ __forceinline__ float2 mul_mi(float2 a, float dir) {
+  return make_float2(a.y * dir, -a.x * dir);
+}
+// Table entry exp(-2 pi i e / L), conjugated for the inverse (dir = -1).
+__device__ __forceinline__ float2 twiddle(const float2* __restrict__ tw,
+                                          int e, float dir) {
+  float2 w = tw[e];
+  w.y *= dir;
+  return w;
+}
+
+struct UpdateSums {
+  float x = 0.f, y = 0.f, u = 0.f, r = 0.f, s = 0.f;
+};
+
+// One coefficient of the fused Y/U update for X = x (already divided by N):
+//   Xr = alpha X + (1 - alpha) Yold                (cbpdn_kernels.cu:657-689)
+//   V  = Xr + U
+//   Y  = copysign(max(|V| - (lambda/rho) w, 0), V)  (cbpdn_kernels.cu:129-311)
+//   AMS map 0: Y = (1 - W) V                       (cbpdn_kernels.cu:1137-1155)
+//   U  = V - Y
+// plus the five residual sums of cbpdn_kernels.cu:695-783 (X unrelaxed,
+// Yprv = Yold, so no copy of the previous Y is kept).  Returns Y - U, the
+// input of the next iteration's forward transform.
+__device__ __forceinline__ float update_point(const Geom& g, const Params& p,
+                                              const Buffers& b, float x,
+                                              float lr, int m, int r, int n,
+                                              long long gi, UpdateSums& acc) {
+  const float yo = b.Y[gi];
+  const float uo = b.U[gi];
+  const float xr = (p.alpha == 1.f) ? x : p.alpha * x + (1.f - p.alpha) * yo;
+  const float v = xr + uo;
+  float yn;
+  if (m < p.m0) {
+    yn = (float)(1 - b.mask[(long long)r * g.C + n]) * v;
+  } else {
+    const float w = p.l1_mode == 0 ? p.l1_scalar
+                                   : (p.l1_mode == 1 ? b.l1w[m] : b.l1w[gi]);
+    const float a = fabsf(v) - lr * w;
+    yn = copysignf(fmaxf(a, 0.f), v);
+    if (p.nonneg) yn = fmaxf(yn, 0.f);
+    if (p.nobndry && (r > g.R - p.Kh || n > g.C - p.Kw)) yn = 0.f;
+  }
+  const float un = v - yn;
+  b.Y[gi] = yn;
+  b.U[gi] = un;
+  acc.x += x * x;
+  acc.y += yn * yn;
+  acc.u += un * un;
+  const float dxy = x - yn;
+  acc.r += dxy * dxy;
+  const float dy = yo - yn;
+  acc.s += dy * dy;
+  return yn - un;
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// Reduce the five sums over the block (fixed order, deterministic) and
+// store them to partials[j * nblocks + bid].  red must hold 5 x nwaves.
+template <int NW>
+__device__ __forceinline__ void block_sums_store(const UpdateSums& acc,
+                                                 float (*red)[NW],
+                                                 float* partials, int nblocks,
+                                                 int bid, int tid) {
+  const int lane = tid & 63, wv = tid >> 6;
+  const float v[5] = {acc.x, acc.y, acc.u, acc.r, acc.s};
+  for (int j = 0; j < 5; ++j) {
+    const float s = wave_sum(v[j]);
+    if (lane == 0) red[j][wv] = s;
+  }
+  __syncthreads();
+  if (tid < 5) {
+    float s = 0.f;
+    for (int w = 0; w < NW; ++w) s += red[tid][w];
+    partials[tid * nblocks + bid] = s;
+  }
+}
+
+// Residuals, stopping test and AutoRho from the five global sums
+// (reference host code cbpdn.cu:405-436 and 521-559, with sporco's
+// sqrt(Nc) absolute tolerance term and guarded normalisers).  Runs on one
+// thread.
+__device__ inline void control_update(const Buffers& b, const Params& p,
+                                      const double* s5, int iter) {
+  Ctrl* c = b.ctrl;
+  const double rho = c->rho;
+  const double nX = sqrt(s5[0]), nY = sqrt(s5[1]), nU = sqrt(s5[2]);
+  double r = sqrt(s5[3]);
+  double s = rho * sqrt(s5[4]);
+  double rn = fmax(nX, nY), sn = rho * nU, epri, edua;
+  if (p.std_residuals) {
+    epri = p.sqrt_nc * p.abs_tol + rn * p.rel_tol;
+    edua = p.sqrt_nc * p.abs_tol + sn * p.rel_tol;
+  } else {
+    if (rn == 0.0) rn = 1.0;
+    if (sn == 0.0) sn = 1.0;
+    r /= rn;
+    s /= sn;
+    epri = p.sqrt_nc * p.abs_tol / rn + p.rel_tol;
+    edua = p.sqrt_nc * p.abs_tol / sn + p.rel_tol;
+  }
+  c->iter = iter;
+  c->r = (float)r;
+  c->s = (float)s;
+  c->epri = (float)epri;
+  c->edua = (float)edua;
+  c->nX = (float)nX;
+  c->nY = (float)nY;
+  c->nU = (float)nU;
+  if (p.track && b.hist != nullptr) {
+    b.hist[3 * (iter - 1) + 0] = (float)r;
+    b.hist[3 * (iter - 1) + 1] = (float)s;
+    b.hist[3 * (iter - 1) + 2] = (float)rho;
+  }
+  float rsf_out = 1.f;
+  if (p.autorho && iter > 1 && iter % p.period == 0) {
+    double rhomlt;
+    if (p.autoscaling) {
+      if (s == 0.0 || r == 0.0) {
+        rhomlt = p.tau;
+      } else {
+        const double q = r / (s * p.xi);
+        rhomlt = sqrt(q >= 1.0 ? q : 1.0 / q);
+        if (rhomlt > p.tau) rhomlt = p.tau;
+      }
+    } else {
+      rhomlt = p.tau;
+    }
+    double rsf = 1.0;
+    if (r > p.xi * p.rsdl_ratio * s)
+      rsf = rhomlt;
+    else if (s > (p.rsdl_ratio / p.xi) * r)
+      rsf = 1.0 / rhomlt;
+    rsf_out = (float)rsf;
+    c->rho = c->rho * rsf_out;
+  }
+  c->rsf = rsf_out;
+  if (r < epri && s < edua) c->done = 1;
+}
+
+}  // namespace scc

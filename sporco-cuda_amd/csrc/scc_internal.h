@@ -1,0 +1,124 @@
+// Internal types shared by the HIP kernels and the host driver.
+//
+// Frequency-domain data is kept in a "panel" layout: the half spectrum of a
+// map, R rows x H = C/2+1 columns, is cut into panels of TC adjacent columns;
+// each panel is stored row-major (R x width) and panels follow each other.
+// A column transform therefore reads one contiguous block, and a row
+// transform touches TC*8 contiguous bytes per panel per row.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace scc {
+
+constexpr int kMaxStages = 24;
+
+// Mixed-radix Stockham plan for one transform length.
+struct FftPlan {
+  int L;                    // transform length (complex points)
+  int nstages;
+  int radix[kMaxStages];
+  int tw_stride;            // twiddle table holds W_{L*tw_stride}^t
+};
+
+struct Geom {
+  int R, C, M;              // image rows, columns, number of maps
+  int H;                    // C/2 + 1
+  int Lr;                   // row complex FFT length: C/2 (C even) or C
+  int even;                 // C even: rows use the half-length packing
+  int TC;                   // panel width
+  int P;                    // number of panels
+  int wlast;                // width of the last panel
+  long long N;              // R*C
+  long long Hc;             // R*H
+};
+
+// Offset of (r, c) of the half spectrum inside one map's panel layout.
+__host__ __device__ inline long long panel_off(const Geom& g, int r, int c) {
+  int p = c / g.TC;
+  int w = (p == g.P - 1) ? g.wlast : g.TC;
+  return (long long)p * g.TC * g.R + (long long)r * w + (c - p * g.TC);
+}
+
+// Device-resident solver state read and written by the kernels, so that an
+// iteration never needs the host.
+struct Ctrl {
+  float rho;       // current penalty parameter
+  float rsf;       // factor of the last rho update (U is divided by it)
+  int done;        // stopping test passed: every later kernel returns early
+  int iter;        // iterations completed
+  float r, s, epri, edua;
+  float nX, nY, nU;
+  float pad_;
+};
+
+// Per-solve constants of the update and control kernels.
+struct Params {
+  float lambda;
+  float mu;
+  float alpha;          // RelaxParam
+  float abs_tol, rel_tol;
+  float tau;            // AutoRho Scaling
+  float rsdl_ratio;     // AutoRho RsdlRatio
+  float xi;             // AutoRho RsdlTarget
+  float l1_scalar;
+  float gw_scalar;
+  double sqrt_nc;       // sqrt(number of coefficients)
+  int std_residuals;
+  int autorho, autoscaling, period;
+  int nonneg, nobndry, Kh, Kw;
+  int m0;               // 1 when map 0 is the AMS impulse map
+  int l1_mode;          // 0 scalar, 1 per map, 2 full (M,R,C) array
+  int grad;             // gradient regularisation on
+  int gw_mode;          // 0 scalar, 1 per map
+  int track;            // record per-iteration history
+};
+
+// Row-kernel modes
+enum RowMode {
+  kRowFused = 0,    // inverse row FFT + update + residuals + forward row FFT
+  kRowUpdate = 1,   // inverse row FFT + update + residuals only
+};
+enum RowFwdMode {
+  kFwdReal = 0,     // forward row FFT of a real (rows, C) array
+  kFwdYU = 1,       // forward row FFT of Y - U/rsf (U rescaled in place)
+};
+
+struct Buffers {
+  float* Y;
+  float* U;
+  float2* F;        // M maps, panel layout: A-hat / X-hat work spectrum
+  float2* Df;       // M maps, panel layout: filter spectra
+  float2* Sf;       // 1 map, panel layout: image spectrum
+  float* ghg;       // 1 map, panel layout (GradReg only)
+  const float* l1w; // per map (M) or full (M,R,C) L1 weights
+  const float* gw;  // per map gradient weights (M)
+  const int* mask;  // (R,C) AMS mask
+  float* partials;  // 5 x nblocks residual partial sums
+  Ctrl* ctrl;
+  float* hist;      // 3 x max_iter history (r, s, rho), optional
+  float2* twR;      // W_R^t, t < R
+  float2* twC;      // W_C^t, t < C
+};
+
+// Kernel launchers (kernels.hip).
+hipError_t launch_pad(const float* D, float* out, int M, int Kh, int Kw,
+                      int R, int C, hipStream_t st);
+hipError_t launch_row_fwd(const Geom& g, int nmaps, const FftPlan& rp,
+                          int mode, const float* src, float* Y, float* U,
+                          float2* F, const float2* twC, const Ctrl* ctrl,
+                          hipStream_t st);
+hipError_t launch_col(const Geom& g, int nmaps, const FftPlan& cp,
+                      float2* F, const float2* twR, int inverse,
+                      const Ctrl* ctrl, hipStream_t st);
+hipError_t launch_solve(const Geom& g, const Buffers& b, const Params& p,
+                        hipStream_t st);
+hipError_t launch_row(const Geom& g, const FftPlan& rp, const Buffers& b,
+                      const Params& p, int mode, int nblocks_out[1],
+                      hipStream_t st);
+hipError_t launch_reduce(const Buffers& b, const Params& p, int nblocks,
+                         int iter, int max_iter, hipStream_t st);
+int row_blocks(const Geom& g);
+
+}  // namespace scc

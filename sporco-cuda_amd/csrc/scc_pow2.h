@@ -1,0 +1,30 @@
+// Register-resident kernels for power-of-two image sizes (kernels_pow2.hip).
+#pragma once
+
+#include "scc_internal.h"
+
+namespace scc {
+
+struct Pow2Config {
+  int TC;        // panel width
+  int rows;      // rows per row-kernel block
+  int variant;   // kernel family index
+};
+
+// Chooses the power-of-two kernels for (R, C, M); false when the generic
+// kernels must be used.
+bool pow2_select(int R, int C, int M, Pow2Config* cfg);
+int pow2_row_blocks(const Geom& g, const Pow2Config& cfg);
+hipError_t pow2_row_fwd(const Geom& g, const Pow2Config& cfg, int nmaps,
+                        int mode, const float* src, float* Y, float* U,
+                        float2* F, const float2* twC, const Ctrl* ctrl,
+                        hipStream_t st);
+hipError_t pow2_col(const Geom& g, const Pow2Config& cfg, int nmaps,
+                    float2* F, const float2* twR, int inverse,
+                    const Ctrl* ctrl, hipStream_t st);
+hipError_t pow2_solve(const Geom& g, const Pow2Config& cfg, const Buffers& b,
+                      const Params& p, hipStream_t st);
+hipError_t pow2_row(const Geom& g, const Pow2Config& cfg, const Buffers& b,
+                    const Params& p, int mode, int* nblocks, hipStream_t st);
+
+}  // namespace scc

@@ -1,0 +1,165 @@
+"""C ABI and drop-in API checks that need no GPU.
+
+The library must load and export every function include/sporco_amd.h
+declares; option defaults must equal default_opts (utils.cu:12-64);
+invalid input must be rejected with a status code (never a process exit)
+before any device work; the Python layer must keep the reference's argument
+errors (Cython buffer checks, KeyError on a missing option key).
+"""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, 'include', 'sporco_amd.h')
+
+
+def declared_functions():
+    txt = open(HEADER).read()
+    txt = re.sub(r'/\*.*?\*/', '', txt, flags=re.S)
+    return sorted(set(re.findall(r'\b(scc_[a-z_]+)\s*\(', txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    from sporco_cuda import _abi
+    L = _abi.lib()
+    names = declared_functions()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(L, n), n
+
+
+def test_struct_layout_matches_header():
+    """The ctypes mirror must have the header's field order and size."""
+    from sporco_cuda import _abi
+    txt = open(HEADER).read()
+    body = re.search(r'typedef struct scc_algopt \{(.*?)\} scc_algopt;', txt,
+                     re.S).group(1)
+    body = re.sub(r'/\*.*?\*/', '', body, flags=re.S)
+    fields = re.findall(r'\**\s*([A-Za-z_0-9]+)\s*;', body)
+    assert fields == [f for f, _ in _abi.AlgOpt._fields_]
+    # Offsets as the C compiler lays the struct out.
+    import shutil
+    import subprocess
+    import tempfile
+    if shutil.which('gcc') is None:
+        pytest.skip('no C compiler')
+    src = '#include <stdio.h>\n#include <stddef.h>\n#include "sporco_amd.h"\n'
+    src += 'int main(void){printf("%zu\\n", sizeof(scc_algopt));'
+    for f in fields:
+        src += 'printf("%%zu\\n", offsetof(scc_algopt, %s));' % f
+    src += 'return 0;}\n'
+    with tempfile.TemporaryDirectory() as d:
+        cf = os.path.join(d, 'l.c')
+        open(cf, 'w').write(src)
+        exe = os.path.join(d, 'l')
+        subprocess.check_call(['gcc', '-I', os.path.dirname(HEADER), cf,
+                               '-o', exe])
+        out = [int(v) for v in subprocess.check_output([exe]).split()]
+    assert out[0] == ctypes.sizeof(_abi.AlgOpt)
+    assert out[1:] == [getattr(_abi.AlgOpt, f).offset for f in fields]
+
+
+def test_default_opts_match_reference_defaults():
+    from sporco_cuda import _abi
+    o = _abi.AlgOpt()
+    L = _abi.lib()
+    L.scc_clear_opts(ctypes.byref(o))
+    assert o.rho == -1 and o.MaxMainIter == -1 and o.device == -1
+    L.scc_default_opts(ctypes.byref(o))
+    assert o.device == 0 and o.Verbose == 0 and o.AutoRho == 1
+    assert o.nWeight == 1 and o.AbsStopTol == 0
+    assert np.isclose(o.RelaxParam, 1.8) and o.RhoScaling == 100
+    assert np.isclose(o.RelStopTol, 1e-4) and o.MaxMainIter == 250
+    assert np.isclose(o.RhoRsdlRatio, 1.2) and o.AutoRhoPeriod == 1
+    assert o.AutoRhoScaling == 1 and o.StdResiduals == 0
+
+
+def _host_call(**over):
+    from sporco_cuda import _abi
+    R, C, M, K = 8, 7, 2, 3
+    D = np.ones((M, K, K), np.float32)
+    S = np.ones((R, C), np.float32)
+    Y = np.zeros((M, R, C), np.float32)
+    w = np.ones(1, np.float32)
+    o = _abi.make_opt(R, C, M, K, K, 1.0, 5,
+                      l1_ptr=w.ctypes.data_as(ctypes.c_void_p))
+    for k, v in over.items():
+        setattr(o, k, v)
+    rc = _abi.lib().scc_cbpdn(D.ctypes.data, S.ctypes.data, 0.1,
+                              ctypes.byref(o), Y.ctypes.data)
+    return rc, _abi.last_error()
+
+
+@pytest.mark.parametrize('field,value', [
+    ('IMG_ROW_SIZE', 0), ('DICT_M_SIZE', 0), ('DICT_ROW_SIZE', 9),
+    ('rho', 0.0), ('rho', float('inf')), ('MaxMainIter', -3),
+    ('L1_WEIGHT_M_SIZE', 3), ('L1Weight', None), ('nWeight', 5),
+])
+def test_invalid_input_rejected_before_device(field, value):
+    from sporco_cuda import _abi
+    rc, msg = _host_call(**{field: value})
+    assert rc == _abi.SCC_ERR_INVALID, (rc, msg)
+    assert msg
+
+
+def test_unsupported_size_rejected():
+    from sporco_cuda import _abi
+    rc, msg = _host_call(IMG_ROW_SIZE=5000)
+    assert rc == _abi.SCC_ERR_UNSUPPORTED, (rc, msg)
+
+
+def test_no_device_is_a_status_not_an_exit():
+    from sporco_cuda import _abi
+    if _abi.lib().scc_get_device_count() > 0:
+        pytest.skip('a device is present')
+    rc, msg = _host_call()
+    assert rc == _abi.SCC_ERR_NODEVICE
+    import sporco_cuda.cbpdn as cb
+    D = np.ones((3, 3, 2), np.float32)
+    S = np.ones((8, 7), np.float32)
+    with pytest.raises(RuntimeError):
+        cb.cbpdn(D, S, 0.1, cb.default_options({'MaxMainIter': 2}))
+
+
+def test_python_argument_errors():
+    import sporco_cuda.cbpdn as cb
+    D = np.ones((3, 3, 2), np.float32)
+    S = np.ones((8, 7), np.float32)
+    opt = cb.default_options({'MaxMainIter': 2})
+    with pytest.raises(ValueError):
+        cb.cbpdn(D.astype(np.float64), S, 0.1, opt)
+    with pytest.raises(ValueError):
+        cb.cbpdn(D[..., 0], S, 0.1, opt)
+    with pytest.raises(TypeError):
+        cb.cbpdn(None, S, 0.1, opt)
+    bad = dict(opt)
+    del bad['RelaxParam']
+    with pytest.raises(KeyError):
+        cb.cbpdn(D, S, 0.1, bad)
+    bad = dict(opt, L1Weight=np.ones((2, 2), np.float32))
+    with pytest.raises(ValueError):
+        cb.cbpdn(D, S, 0.1, bad)
+
+
+def test_default_options_merge():
+    import sporco_cuda.cbpdn as cb
+    o = cb.default_options({'MaxMainIter': 7, 'AutoRho': {'Enabled': False}},
+                           grad=True)
+    assert o['MaxMainIter'] == 7 and o['GradWeight'] == 1.0
+    assert o['AutoRho']['Enabled'] is False
+    assert o['AutoRho']['Period'] == 1 and o['RelaxParam'] == 1.8
+
+
+def test_device_queries_without_device():
+    import sporco_cuda.util as cu
+    n = cu.device_count()
+    assert n >= 0
+    if n == 0:
+        assert cu.device_name() is None
+        with pytest.raises(RuntimeError):
+            cu.memory_info()
