@@ -1,0 +1,243 @@
+#!/usr/bin/env python
+"""ADMM iterations per second of plain CBPDN at 1024 x 1024 with 64 filters.
+
+One step is one ADMM iteration of the plain CBPDN solver (forward column
+FFTs, Sherman-Morrison X-solve, inverse column FFTs, fused row kernel,
+residual reduction) on a synthetic image and dictionary resident in HBM,
+called through the C ABI's device-pointer entry point.  The timed region is
+one solve of exactly --steps iterations (fixed rho, AutoRho off, zero
+stopping tolerances), so it also contains the solve's setup (spectra of D
+and S), which makes the figure slightly conservative.
+
+The JSON line carries:
+  roofline      the dominant kernel's algorithmic bytes per launch over its
+                average duration (HIP events on the solve stream), against
+                the 8 TB/s HBM peak; traffic from the committed rocprofv3
+                counter summary when one exists for the workload;
+  cpu_baseline  the FP32 numpy/scipy restatement of the same loop (the
+                oracle) timed on the host for a bounded number of iterations;
+  iteration_model  the whole iteration against the 4-stage compulsory
+                traffic model 32 M N + 40 M Hc + 12 Hc bytes.
+"""
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, 'sporco-cuda_amd'))
+
+METRIC = ('ADMM iters/s at 1024²×64 filters (% HBM roofline); '
+          'batched/adjoint vs plain')
+HBM_PEAK_GBPS = 8000.0
+
+WORKLOADS = {
+    'plain-1024x1024x64': dict(H=1024, W=1024, K=8, M=64, lmbda=0.05,
+                               rho=50.0, relax=1.8, seed=5),
+    'plain-256x256x32': dict(H=256, W=256, K=8, M=32, lmbda=0.05, rho=50.0,
+                             relax=1.8, seed=0),
+    'plain-2048x2048x128': dict(H=2048, W=2048, K=16, M=128, lmbda=0.02,
+                                rho=100.0, relax=1.8, seed=2),
+}
+
+
+def synth_image(H, W, seed):
+    """Sum of 20 Gaussian blobs + 0.1 N(0,1), normalised to [0,1], minus a
+    circular 5x5 box mean."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:H, 0:W].astype(np.float64)
+    img = np.zeros((H, W))
+    smin = min(H, W)
+    for _ in range(20):
+        cy, cx = rng.uniform(0, H), rng.uniform(0, W)
+        sg = rng.uniform(smin / 32.0, smin / 8.0)
+        amp = rng.uniform(0.5, 1.0)
+        img += amp * np.exp(-((yy - cy) ** 2 + (xx - cx) ** 2) / (2 * sg * sg))
+    img += 0.1 * rng.standard_normal((H, W))
+    img = (img - img.min()) / (img.max() - img.min())
+    box = np.zeros_like(img)
+    for a in range(-2, 3):
+        for b in range(-2, 3):
+            box += np.roll(img, (a, b), axis=(0, 1))
+    return (img - box / 25.0).astype(np.float32)
+
+
+def synth_dict(K, M, seed):
+    """i.i.d. N(0,1) K x K filters, unit L2 norm each, as (M, K, K)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    D = rng.standard_normal((K, K, M))
+    D /= np.sqrt(np.sum(D ** 2, axis=(0, 1), keepdims=True))
+    return np.ascontiguousarray(np.rollaxis(D.astype(np.float32), 2, 0))
+
+
+def model_bytes_iter(M, R, C):
+    """Compulsory fp32 bytes of one iteration of the 4-stage form."""
+    N = R * C
+    Hc = R * (C // 2 + 1)
+    return 32 * M * N + 40 * M * Hc + 12 * Hc
+
+
+def stage_bytes(M, R, C):
+    """Algorithmic bytes per launch of each kernel of the iteration."""
+    N = R * C
+    Hc = R * (C // 2 + 1)
+    return {
+        'col_fwd': 16 * M * Hc,                 # read + write the spectra
+        'solve': 24 * M * Hc + 8 * Hc,          # A, D spectra in, X out, S
+        'col_inv': 16 * M * Hc,
+        'row': 16 * M * N + 16 * M * Hc,        # X in, Y U in/out, A out
+    }
+
+
+def cpu_baseline(wl, D, S, budget_s):
+    """FP32 oracle loop on the host cores for about budget_s seconds."""
+    import numpy as np
+    from oracle import cbpdn as ocb
+    cores = len(os.sched_getaffinity(0))
+    stamps = []
+
+    class _Stop(Exception):
+        pass
+
+    def cb(i, X, Y, U):
+        stamps.append(time.perf_counter())
+        if stamps[-1] - stamps[0] > budget_s:
+            raise _Stop()
+    try:
+        ocb.solve(D, S, wl['lmbda'], rho=wl['rho'], max_iter=10000,
+                  relax_param=wl['relax'], rel_tol=0.0, dtype=np.float32,
+                  workers=cores, callback=cb)
+    except _Stop:
+        pass
+    n = len(stamps) - 1
+    per_it = (stamps[-1] - stamps[0]) / max(n, 1)
+    return {
+        'value': 1.0 / per_it, 'unit': 'ADMM iterations/s', 'cores': cores,
+        'kind': 'port',
+        'sample': ('%d iterations of the same workload (after the first), '
+                   'FP32 numpy + scipy.fft with workers=%d; numpy '
+                   'elementwise work single-threaded') % (n, cores),
+    }
+
+
+def load_traffic(workload, stage):
+    path = os.path.join(ROOT, 'profiles', 'traffic.json')
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get(workload, {}).get(stage)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=200)
+    ap.add_argument('--warmup', type=int, default=20)
+    ap.add_argument('--workload', default='plain-1024x1024x64',
+                    choices=sorted(WORKLOADS))
+    ap.add_argument('--cpu-seconds', type=float, default=15.0)
+    ap.add_argument('--no-cpu', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        sys.stderr.write('bench.py measures one GPU: this build runs the '
+                         'solver on a single MI355X (multi-GPU sharding is '
+                         'design-only); use --gpus 1\n')
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from sporco_cuda import _abi
+
+    wl = WORKLOADS[args.workload]
+    R, C, M, K = wl['H'], wl['W'], wl['M'], wl['K']
+    S_h = synth_image(R, C, 1000 + wl['seed'])
+    D_h = synth_dict(K, M, 2000 + wl['seed'])
+    dev = torch.device('cuda:0')
+    D = torch.from_numpy(D_h).to(dev)
+    S = torch.from_numpy(S_h).to(dev)
+    w = torch.ones(1, dtype=torch.float32, device=dev)
+    Y = torch.empty((M, R, C), dtype=torch.float32, device=dev)
+    L = _abi.lib()
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def solve(iters):
+        o = _abi.make_opt(R, C, M, K, K, wl['rho'], iters, relax=wl['relax'],
+                          l1_ptr=w.data_ptr())
+        rc = L.scc_cbpdn_device(D.data_ptr(), S.data_ptr(), wl['lmbda'], 0.0,
+                                0, ctypes.byref(o), Y.data_ptr(), stream)
+        if rc != 0:
+            raise RuntimeError('solve failed (%d): %s' % (rc,
+                                                          _abi.last_error()))
+
+    print('bench: %s warmup %d iterations' % (args.workload, args.warmup),
+          file=sys.stderr, flush=True)
+    solve(args.warmup)
+    torch.cuda.synchronize()
+    print('bench: timing %d iterations' % args.steps, file=sys.stderr,
+          flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solve(args.steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    info = _abi.last_info()
+    value = args.steps / (t1 - t0)
+
+    # Per-kernel durations over a second run of the same length.
+    L.scc_set_stage_timing(1)
+    solve(args.steps)
+    torch.cuda.synchronize()
+    L.scc_set_stage_timing(0)
+    st = _abi.last_stage_times()
+    sb = stage_bytes(M, R, C)
+    per_kernel = {}
+    for k, (ms, n) in st.items():
+        if n and k in sb:
+            avg = ms / n
+            per_kernel[k] = {'avg_us': 1000 * avg,
+                             'GBps': sb[k] / (avg * 1e-3) / 1e9}
+    dom = max(per_kernel, key=lambda k: st[k][0])
+    ach = per_kernel[dom]['GBps']
+    roofline = {'bound': 'hbm', 'achieved': ach, 'peak': HBM_PEAK_GBPS,
+                'unit': 'GB/s', 'frac': ach / HBM_PEAK_GBPS,
+                'traffic': load_traffic(args.workload, dom),
+                'kernel': dom, 'algorithmic_bytes': sb[dom]}
+    mb = model_bytes_iter(M, R, C)
+    out = {
+        'metric': METRIC, 'value': value, 'unit': 'ADMM iterations/s',
+        'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': 1000.0 / value, 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32',
+        'data': 'synthetic (seeded Gaussian-blob image, unit-norm Gaussian '
+                'dictionary)',
+        'config': {'workload': '%s: plain CBPDN, %dx%d image, %d filters '
+                               '%dx%d, lambda=%g, rho=%g, RelaxParam=%g, '
+                               'AutoRho off, fixed iterations'
+                               % (args.workload, R, C, M, K, K, wl['lmbda'],
+                                  wl['rho'], wl['relax'])},
+        'roofline': roofline,
+        'iteration_model': {'bytes_per_iter': mb,
+                            'achieved_GBps': mb * value / 1e9,
+                            'frac': mb * value / 1e9 / HBM_PEAK_GBPS,
+                            'device_solve_ms_per_iter':
+                                info['solve_ms'] / max(info['iterations'], 1)},
+        'kernels': per_kernel,
+        'setup_ms': info['setup_ms'],
+    }
+    if not args.no_cpu:
+        print('bench: cpu baseline (%.0f s budget)' % args.cpu_seconds,
+              file=sys.stderr, flush=True)
+        out['cpu_baseline'] = cpu_baseline(wl, D_h, S_h, args.cpu_seconds)
+    else:
+        out['cpu_baseline'] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -124,6 +124,16 @@ const char *scc_last_error(void);
 void scc_set_history(int enabled);
 int scc_last_history(float *rsr, int cap);
 
+/* Per-stage device timing of later solves on this thread (HIP events on
+ * the solve stream around each kernel of every iteration).  Stages:
+ * 0 forward column FFT, 1 X-solve, 2 inverse column FFT, 3 row kernel
+ * (inverse row FFT + update + forward row FFT), 4 residual reduction and
+ * control, 5 forward row FFT after a rho change.  scc_last_stage_times
+ * fills total milliseconds and launch counts for up to n stages and
+ * returns the number of stages.  No reference counterpart. */
+void scc_set_stage_timing(int enabled);
+int scc_last_stage_times(double *ms_total, int *launches, int n);
+
 /* Device queries: replace get_device_count, get_current_device,
  * set_current_device, get_memory_info, get_device_name
  * (sporco_cuda/src/utils.cu:146-175, bound at util.pyx:3-7). */

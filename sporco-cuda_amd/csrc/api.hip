@@ -35,6 +35,13 @@ thread_local std::string g_err;
 thread_local scc_info g_info;
 thread_local bool g_hist_on = false;
 thread_local std::vector<float> g_hist;
+// Per-stage device timing of the ADMM iterations (events on the solve
+// stream around every kernel): index = Stage.
+enum Stage { kStColFwd = 0, kStSolve, kStColInv, kStRow, kStReduce, kStRowFwd,
+             kNumStages };
+thread_local bool g_stage_on = false;
+thread_local double g_stage_ms[kNumStages];
+thread_local int g_stage_n[kNumStages];
 
 int fail(int code, const std::string& msg) {
   g_err = msg;
@@ -421,26 +428,58 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
                          hipMemcpyHostToDevice, st));
   SCC_HIP(hipEventRecord(ws->ev[1], st));
 
+  // Stage timing: one event before the first stage of an iteration and one
+  // after every stage.
+  std::vector<hipEvent_t> sev;
+  std::vector<int> sid;
+  for (int k = 0; k < kNumStages; ++k) {
+    g_stage_ms[k] = 0.0;
+    g_stage_n[k] = 0;
+  }
+  auto mark = [&](int stage) -> hipError_t {
+    if (!g_stage_on) return hipSuccess;
+    hipEvent_t e;
+    hipError_t err = hipEventCreate(&e);
+    if (err != hipSuccess) return err;
+    sev.push_back(e);
+    sid.push_back(stage);
+    return hipEventRecord(e, st);
+  };
+  struct EvFree {
+    std::vector<hipEvent_t>* v;
+    ~EvFree() {
+      for (auto e : *v) (void)hipEventDestroy(e);
+    }
+  } evfree{&sev};
+
   // A stopping test can pass only with a positive tolerance.
   const bool can_stop = o.RelStopTol > 0.f || o.AbsStopTol > 0.f;
   const int poll = o.Verbose == 1 ? 1 : 16;
   int nblocks = 0;
   for (int i = 1; i <= maxit; ++i) {
     const bool rho_step = p.autorho && i > 1 && i % p.period == 0;
+    const int rmode = rho_step ? kRowUpdate : kRowFused;
+    SCC_HIP(mark(-1));
     if (ws->pow2) {
       SCC_HIP(pow2_col(g, ws->p2, M, b.F, b.twR, 0, b.ctrl, st));
+      SCC_HIP(mark(kStColFwd));
       SCC_HIP(pow2_solve(g, ws->p2, b, p, st));
+      SCC_HIP(mark(kStSolve));
       SCC_HIP(pow2_col(g, ws->p2, M, b.F, b.twR, 1, b.ctrl, st));
-      SCC_HIP(pow2_row(g, ws->p2, b, p, rho_step ? kRowUpdate : kRowFused,
-                       &nblocks, st));
+      SCC_HIP(mark(kStColInv));
+      SCC_HIP(pow2_row(g, ws->p2, b, p, rmode, &nblocks, st));
     } else {
       SCC_HIP(launch_col(g, M, ws->colp, b.F, b.twR, 0, b.ctrl, st));
+      SCC_HIP(mark(kStColFwd));
       SCC_HIP(launch_solve(g, b, p, st));
+      SCC_HIP(mark(kStSolve));
       SCC_HIP(launch_col(g, M, ws->colp, b.F, b.twR, 1, b.ctrl, st));
-      SCC_HIP(launch_row(g, ws->rowp, b, p, rho_step ? kRowUpdate : kRowFused,
-                         &nblocks, st));
+      SCC_HIP(mark(kStColInv));
+      SCC_HIP(launch_row(g, ws->rowp, b, p, rmode, &nblocks, st));
     }
+    SCC_HIP(mark(kStRow));
     SCC_HIP(launch_reduce(b, p, nblocks, i, maxit, st));
+    SCC_HIP(mark(kStReduce));
     if (rho_step) {
       if (ws->pow2)
         SCC_HIP(pow2_row_fwd(g, ws->p2, M, kFwdYU, nullptr, b.Y, b.U, b.F,
@@ -448,6 +487,7 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
       else
         SCC_HIP(launch_row_fwd(g, M, ws->rowp, kFwdYU, nullptr, b.Y, b.U, b.F,
                                b.twC, b.ctrl, st));
+      SCC_HIP(mark(kStRowFwd));
     }
     if ((can_stop || o.Verbose == 1) && (i % poll == 0 || i == maxit)) {
       SCC_HIP(hipMemcpyAsync(ws->hctrl, b.ctrl, sizeof(Ctrl),
@@ -471,6 +511,13 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
   SCC_HIP(hipMemcpyAsync(ws->hctrl, b.ctrl, sizeof(Ctrl),
                          hipMemcpyDeviceToHost, st));
   SCC_HIP(hipStreamSynchronize(st));
+  for (size_t k = 1; k < sev.size(); ++k) {
+    if (sid[k] < 0) continue;
+    float ms = 0.f;
+    SCC_HIP(hipEventElapsedTime(&ms, sev[k - 1], sev[k]));
+    g_stage_ms[sid[k]] += ms;
+    g_stage_n[sid[k]] += 1;
+  }
   const Ctrl& cf = *ws->hctrl;
   g_info = scc_info{};
   g_info.iterations = cf.iter;
@@ -645,6 +692,16 @@ int scc_last_history(float* rsr, int cap) {
   return n;
 }
 
+void scc_set_stage_timing(int enabled) { g_stage_on = enabled != 0; }
+
+int scc_last_stage_times(double* ms_total, int* launches, int n) {
+  for (int k = 0; k < n && k < kNumStages; ++k) {
+    if (ms_total) ms_total[k] = g_stage_ms[k];
+    if (launches) launches[k] = g_stage_n[k];
+  }
+  return kNumStages;
+}
+
 int scc_get_device_count(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;
@@ -679,6 +736,9 @@ const char* scc_get_device_name(int dev) {
   static thread_local hipDeviceProp_t prop;
   if (check_device(dev)) return nullptr;
   if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return nullptr;
+  // Some driver stacks leave the marketing name empty; the architecture
+  // name still identifies the device.
+  if (prop.name[0] == '\0') return prop.gcnArchName;
   return prop.name;
 }
 

@@ -77,6 +77,8 @@ def lib():
                                      ctypes.POINTER(ctypes.c_size_t)]),
         'scc_get_device_name': (ctypes.c_char_p, [_i]),
         'scc_release_cache': (None, []),
+        'scc_set_stage_timing': (None, [_i]),
+        'scc_last_stage_times': (_i, [vp, vp, _i]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -122,3 +124,15 @@ def last_info():
     inf = Info()
     lib().scc_last_info(ctypes.byref(inf))
     return {k: getattr(inf, k) for k, _ in Info._fields_}
+
+
+STAGES = ('col_fwd', 'solve', 'col_inv', 'row', 'reduce', 'row_fwd')
+
+
+def last_stage_times():
+    """{stage: (total ms, launches)} of the last solve with stage timing."""
+    import numpy as np
+    ms = np.zeros(len(STAGES), np.float64)
+    n = np.zeros(len(STAGES), np.int32)
+    lib().scc_last_stage_times(ms.ctypes.data, n.ctypes.data, len(STAGES))
+    return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(STAGES)}
