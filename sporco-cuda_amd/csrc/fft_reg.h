@@ -1,0 +1,243 @@
+// Register-resident Stockham FFT of power-of-two length L for gfx950.
+//
+// T = L / E threads cooperate on one transform; thread t holds the E
+// elements t + T e (e < E) in registers.  Every stage uses radix RS <= E
+// (RS divides E), so the inputs of a thread's butterflies are its own
+// registers: butterfly j = t + T b reads elements j + r L/RS, i.e. slots
+// b + r E/RS.  Outputs go to their natural Stockham positions in LDS and are
+// read back into slots t + T e, so one stage costs one LDS write and one LDS
+// read per element.  Stage radices: E while E divides what is left, then the
+// remainder, e.g. L = 1024, E = 16 -> 16, 16, 4.
+//
+// The phases (compute, write, read) are __host__ __device__ so that the
+// index and twiddle arithmetic is tested on the CPU with the barriers
+// replaced by loops over threads.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace scc {
+namespace reg {
+
+__host__ __device__ __forceinline__ float2 c_add(float2 a, float2 b) {
+  return make_float2(a.x + b.x, a.y + b.y);
+}
+__host__ __device__ __forceinline__ float2 c_sub(float2 a, float2 b) {
+  return make_float2(a.x - b.x, a.y - b.y);
+}
+__host__ __device__ __forceinline__ float2 c_mul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * (-i)^DIR-ish: multiply by -i for the forward (DIR = 1) transform,
+// by +i for the inverse (DIR = -1).
+template <int DIR>
+__host__ __device__ __forceinline__ float2 c_mul_mi(float2 a) {
+  return DIR > 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+
+// cos(2 pi m / 16), m = 0..15.
+__host__ __device__ constexpr float kCos16(int m) {
+  return m == 0 ? 1.0f
+       : m == 1 ? 0.92387953251128674f
+       : m == 2 ? 0.70710678118654752f
+       : m == 3 ? 0.38268343236508979f
+       : m == 4 ? 0.0f
+       : m == 5 ? -0.38268343236508979f
+       : m == 6 ? -0.70710678118654752f
+       : m == 7 ? -0.92387953251128674f
+       : m == 8 ? -1.0f
+       : m == 9 ? -0.92387953251128674f
+       : m == 10 ? -0.70710678118654752f
+       : m == 11 ? -0.38268343236508979f
+       : m == 12 ? 0.0f
+       : m == 13 ? 0.38268343236508979f
+       : m == 14 ? 0.70710678118654752f
+       : 0.92387953251128674f;
+}
+// exp(-2 pi i DIR m / N) for N dividing 16.
+template <int N, int DIR>
+__host__ __device__ constexpr float2 w_const(int m) {
+  return make_float2(kCos16(((m * (16 / N)) % 16)),
+                     (float)(-DIR) * kCos16((((m * (16 / N)) % 16) + 12) % 16));
+}
+
+// In-place DFT of N = 2, 4, 8 or 16 points, forward (DIR = 1) or
+// unnormalised inverse (DIR = -1).  N = 8, 16 use the split
+// N = 4 * (N / 4): inner 4-point DFTs, constant twiddles, outer DFTs.
+template <int N, int DIR>
+struct Dft;
+
+template <int DIR>
+struct Dft<2, DIR> {
+  __host__ __device__ __forceinline__ static void run(float2* x) {
+    const float2 a = x[0], b = x[1];
+    x[0] = c_add(a, b);
+    x[1] = c_sub(a, b);
+  }
+};
+
+template <int DIR>
+struct Dft<4, DIR> {
+  __host__ __device__ __forceinline__ static void run(float2* x) {
+    const float2 s02 = c_add(x[0], x[2]), d02 = c_sub(x[0], x[2]);
+    const float2 s13 = c_add(x[1], x[3]), d13 = c_mul_mi<DIR>(c_sub(x[1], x[3]));
+    x[0] = c_add(s02, s13);
+    x[1] = c_add(d02, d13);
+    x[2] = c_sub(s02, s13);
+    x[3] = c_sub(d02, d13);
+  }
+};
+
+template <int N, int DIR>
+struct Dft {
+  static_assert(N == 8 || N == 16, "radix");
+  __host__ __device__ __forceinline__ static void run(float2* x) {
+    constexpr int N2 = N / 4;  // n = N2 n1 + n2, k = k1 + 4 k2
+    float2 y[N];
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) {
+      float2 a[4];
+#pragma unroll
+      for (int n1 = 0; n1 < 4; ++n1) a[n1] = x[N2 * n1 + n2];
+      Dft<4, DIR>::run(a);
+#pragma unroll
+      for (int k1 = 0; k1 < 4; ++k1) {
+        float2 v = a[k1];
+        if (n2 * k1 != 0) v = c_mul(v, w_const<N, DIR>(n2 * k1));
+        y[n2 * 4 + k1] = v;
+      }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      float2 b[N2];
+#pragma unroll
+      for (int n2 = 0; n2 < N2; ++n2) b[n2] = y[n2 * 4 + k1];
+      Dft<N2, DIR>::run(b);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) x[k1 + 4 * k2] = b[k2];
+    }
+  }
+};
+
+// Padded LDS index: one spare slot after every 16 complex values keeps the
+// strided Stockham writes free of bank conflicts.
+__host__ __device__ __forceinline__ constexpr int lpad(int i) {
+  return i + (i >> 4);
+}
+__host__ __device__ constexpr int lds_len(int L) { return L + (L >> 4) + 1; }
+
+template <int L, int E>
+struct Plan {
+  static_assert((L & (L - 1)) == 0 && (E & (E - 1)) == 0 && E <= L, "pow2");
+  static constexpr int T = L / E;
+  static constexpr int stages() {
+    int n = 0, left = L;
+    while (left > 1) {
+      left /= (left >= E ? E : left);
+      ++n;
+    }
+    return n;
+  }
+  static constexpr int radix(int s) {
+    int left = L;
+    for (int i = 0; i < s; ++i) left /= (left >= E ? E : left);
+    return left >= E ? E : left;
+  }
+  static constexpr int ns(int s) {
+    int n = 1;
+    for (int i = 0; i < s; ++i) n *= radix(i);
+    return n;
+  }
+};
+
+// Twiddle W_L^e (table W_{L*tws}^t, t < L*tws), conjugated when DIR < 0.
+template <int DIR>
+__host__ __device__ __forceinline__ float2 tw_get(const float2* tw, int e,
+                                                   int tws) {
+  float2 w = tw[e * tws];
+  if (DIR < 0) w.y = -w.y;
+  return w;
+}
+
+// Stage S: twiddle and butterflies on thread t's registers.
+template <int L, int E, int S, int DIR>
+__host__ __device__ __forceinline__ void stage_compute(float2 (&v)[E], int t,
+                                                       const float2* tw,
+                                                       int tws) {
+  using P = Plan<L, E>;
+  constexpr int RS = P::radix(S);
+  constexpr int NS = P::ns(S);
+  constexpr int NB = E / RS;
+  constexpr int T = P::T;
+  constexpr int STEP = L / (NS * RS);
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    float2 x[RS];
+#pragma unroll
+    for (int r = 0; r < RS; ++r) x[r] = v[b + r * NB];
+    if (NS > 1) {
+      const int k = (t + T * b) & (NS - 1);
+#pragma unroll
+      for (int r = 1; r < RS; ++r)
+        x[r] = c_mul(x[r], tw_get<DIR>(tw, k * r * STEP, tws));
+    }
+    Dft<RS, DIR>::run(x);
+#pragma unroll
+    for (int q = 0; q < RS; ++q) v[b + q * NB] = x[q];
+  }
+}
+
+// Stage S outputs to their natural positions in the (padded) LDS buffer.
+template <int L, int E, int S>
+__host__ __device__ __forceinline__ void stage_write(const float2 (&v)[E],
+                                                     int t, float2* lds) {
+  using P = Plan<L, E>;
+  constexpr int RS = P::radix(S);
+  constexpr int NS = P::ns(S);
+  constexpr int NB = E / RS;
+  constexpr int T = P::T;
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = t + T * b;
+    const int k = j & (NS - 1);
+    const int ob = (j - k) * RS + k;
+#pragma unroll
+    for (int q = 0; q < RS; ++q) lds[lpad(ob + q * NS)] = v[b + q * NB];
+  }
+}
+
+// Slots t + T e from the LDS buffer (natural order).
+template <int L, int E>
+__host__ __device__ __forceinline__ void slots_read(float2 (&v)[E], int t,
+                                                    const float2* lds) {
+  constexpr int T = L / E;
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = lds[lpad(t + T * e)];
+}
+
+template <int L, int E>
+__host__ __device__ __forceinline__ void slots_write(const float2 (&v)[E],
+                                                     int t, float2* lds) {
+  constexpr int T = L / E;
+#pragma unroll
+  for (int e = 0; e < E; ++e) lds[lpad(t + T * e)] = v[e];
+}
+
+// Whole transform on the device: registers in (slots), result in natural
+// order in lds on return (block barrier included).  Every thread of the
+// block must call it.  Two barriers per stage: LDS is single-buffered.
+template <int L, int E, int DIR, int S = 0>
+__device__ __forceinline__ void fft_dev(float2 (&v)[E], int t, float2* lds,
+                                        const float2* tw, int tws) {
+  stage_compute<L, E, S, DIR>(v, t, tw, tws);
+  stage_write<L, E, S>(v, t, lds);
+  __syncthreads();
+  if constexpr (S + 1 < Plan<L, E>::stages()) {
+    slots_read<L, E>(v, t, lds);
+    __syncthreads();
+    fft_dev<L, E, DIR, S + 1>(v, t, lds, tw, tws);
+  }
+}
+
+}  // namespace reg
+}  // namespace scc
