@@ -198,11 +198,12 @@ def main():
     sb = stage_bytes(M, R, C)
     per_kernel = {}
     for k, (ms, n) in st.items():
-        if n and k in sb:
+        if n:
             avg = ms / n
-            per_kernel[k] = {'avg_us': 1000 * avg,
-                             'GBps': sb[k] / (avg * 1e-3) / 1e9}
-    dom = max(per_kernel, key=lambda k: st[k][0])
+            per_kernel[k] = {'avg_us': 1000 * avg}
+            if k in sb:
+                per_kernel[k]['GBps'] = sb[k] / (avg * 1e-3) / 1e9
+    dom = max((k for k in per_kernel if k in sb), key=lambda k: st[k][0])
     ach = per_kernel[dom]['GBps']
     roofline = {'bound': 'hbm', 'achieved': ach, 'peak': HBM_PEAK_GBPS,
                 'unit': 'GB/s', 'frac': ach / HBM_PEAK_GBPS,

@@ -119,7 +119,7 @@ struct Workspace {
   FftPlan rowp{}, colp{};
   bool pow2 = false;
   Pow2Config p2{};
-  DevBuf Y, U, F, Df, Sf, ghg, partials, ctrl, hist, twR, twC;
+  DevBuf Y, U, F, Df, Sf, ghg, partials, blocksum, ctrl, hist, twR, twC;
   DevBuf sD, sS, sL1, sGW, sMask;  // staging for the host-pointer entry
   bool ghg_ready = false;
   hipStream_t stream = nullptr;
@@ -145,6 +145,10 @@ int make_geom(int R, int C, int M, Geom* g, bool* pow2, Pow2Config* p2) {
   g->Lr = g->even ? C / 2 : C;
   g->N = (long long)R * C;
   g->Hc = (long long)R * g->H;
+  g->lgR = -1;
+  if ((R & (R - 1)) == 0)
+    for (g->lgR = 0; (1 << g->lgR) < R; ++g->lgR) {
+    }
   *pow2 = pow2_select(R, C, M, p2);
   int TC;
   if (*pow2) {
@@ -195,7 +199,9 @@ int get_workspace(int device, int R, int C, int M, Workspace** out) {
       (rc = ensure(ws->twC, (size_t)C * 8)))
     return rc;
   const int nb = ws->pow2 ? pow2_row_blocks(g, ws->p2) : row_blocks(g);
-  if ((rc = ensure(ws->partials, (size_t)5 * nb * 4))) return rc;
+  if ((rc = ensure(ws->partials, (size_t)5 * nb * 4)) ||
+      (rc = ensure(ws->blocksum, (size_t)5 * 64 * 8)))
+    return rc;
   SCC_HIP(hipStreamCreateWithFlags(&ws->stream, hipStreamNonBlocking));
   for (auto& e : ws->ev) SCC_HIP(hipEventCreate(&e));
   SCC_HIP(hipHostMalloc((void**)&ws->hctrl, sizeof(Ctrl)));
@@ -390,6 +396,7 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
   b.gw = gw;
   b.mask = mask;
   b.partials = (float*)ws->partials.p;
+  b.blocksum = (double*)ws->blocksum.p;
   b.ctrl = (Ctrl*)ws->ctrl.p;
   b.hist = p.track ? (float*)ws->hist.p : nullptr;
   b.twR = (float2*)ws->twR.p;

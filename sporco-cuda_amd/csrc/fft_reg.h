@@ -223,6 +223,52 @@ __host__ __device__ __forceinline__ void slots_write(const float2 (&v)[E],
   for (int e = 0; e < E; ++e) lds[lpad(t + T * e)] = v[e];
 }
 
+// The last stage needs no exchange: with NS = L / RS its butterfly j < NS
+// writes elements j + q NS = t + T (b + q NB), i.e. the thread's own slots.
+// So a transform whose last stage is kept in registers leaves element
+// t + T e of the result in v[e].
+
+// Ordering of LDS accesses between the lanes of one wavefront: LDS
+// instructions of a wavefront execute in order, so only the compiler must
+// be kept from moving accesses across this point.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Transform whose T threads all belong to one wavefront: stage exchanges
+// through this transform's private LDS region with wavefront ordering only
+// (no block barrier).  Slots in, slots out (element t + T e in v[e]).
+template <int L, int E, int DIR, int S = 0>
+__device__ __forceinline__ void fft_wave(float2 (&v)[E], int t, float2* lds,
+                                         const float2* tw, int tws) {
+  stage_compute<L, E, S, DIR>(v, t, tw, tws);
+  if constexpr (S + 1 < Plan<L, E>::stages()) {
+    stage_write<L, E, S>(v, t, lds);
+    wave_sync();
+    slots_read<L, E>(v, t, lds);
+    wave_sync();
+    fft_wave<L, E, DIR, S + 1>(v, t, lds, tw, tws);
+  }
+}
+
+// Transform spread over several wavefronts of the block: block barriers
+// around every exchange.  Slots in, slots out.  Every thread of the block
+// must call it.
+template <int L, int E, int DIR, int S = 0>
+__device__ __forceinline__ void fft_block(float2 (&v)[E], int t, float2* lds,
+                                          const float2* tw, int tws) {
+  stage_compute<L, E, S, DIR>(v, t, tw, tws);
+  if constexpr (S + 1 < Plan<L, E>::stages()) {
+    stage_write<L, E, S>(v, t, lds);
+    __syncthreads();
+    slots_read<L, E>(v, t, lds);
+    __syncthreads();
+    fft_block<L, E, DIR, S + 1>(v, t, lds, tw, tws);
+  }
+}
+
 // Whole transform on the device: registers in (slots), result in natural
 // order in lds on return (block barrier included).  Every thread of the
 // block must call it.  Two barriers per stage: LDS is single-buffered.

@@ -379,32 +379,56 @@ __global__ __launch_bounds__(256) void k_row_generic(Geom g, FftPlan rp,
 }
 
 // --------------------------------------------------------------------------
-// Residual norms, stopping test and AutoRho, single block, deterministic.
+// Residual norms, stopping test and AutoRho.  Two deterministic passes:
+// k_reduce_blocks sums contiguous chunks of the partial sums in double (one
+// result per block), k_reduce_final adds the block results in index order
+// and runs the control logic.  The kernel boundary between them makes the
+// block results visible on every XCD.
 // --------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_reduce(Buffers b, Params p,
-                                                 int nblocks, int iter) {
+constexpr int kReduceBlocks = 64;  // <= 64: one lane each in the final pass
+constexpr int kReduceThreads = 1024;
+
+__global__ __launch_bounds__(kReduceThreads) void k_reduce_blocks(
+    Buffers b, int nparts) {
   if (b.ctrl->done) return;
-  __shared__ double red[5][16];
+  __shared__ double red[5][kReduceThreads / 64];
+  const int chunk = (nparts + gridDim.x - 1) / gridDim.x;
+  const int i0 = blockIdx.x * chunk;
+  const int i1 = min(nparts, i0 + chunk);
   double acc[5] = {0, 0, 0, 0, 0};
   const int tid = threadIdx.x;
-  for (int i = tid; i < nblocks; i += blockDim.x)
-    for (int j = 0; j < 5; ++j) acc[j] += (double)b.partials[j * nblocks + i];
+  for (int i = i0 + tid; i < i1; i += kReduceThreads)
+#pragma unroll
+    for (int j = 0; j < 5; ++j) acc[j] += (double)b.partials[j * nparts + i];
   const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
   for (int j = 0; j < 5; ++j) {
     double v = acc[j];
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
     if (lane == 0) red[j][wv] = v;
   }
   __syncthreads();
-  if (tid != 0) return;
-  const int nw = blockDim.x >> 6;
-  double s5[5];
-  for (int j = 0; j < 5; ++j) {
+  if (tid < 5) {
     double v = 0;
-    for (int w = 0; w < nw; ++w) v += red[j][w];
+    for (int w = 0; w < kReduceThreads / 64; ++w) v += red[tid][w];
+    b.blocksum[blockIdx.x * 5 + tid] = v;
+  }
+}
+
+// One wavefront: lane k loads block k's five sums, a fixed shuffle tree
+// adds them (same order every call), lane 0 runs the control logic.
+__global__ __launch_bounds__(64) void k_reduce_final(Buffers b, Params p,
+                                                     int nblk, int iter) {
+  if (b.ctrl->done) return;
+  const int k = threadIdx.x;
+  double s5[5];
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    double v = k < nblk ? b.blocksum[k * 5 + j] : 0.0;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
     s5[j] = v;
   }
-  control_update(b, p, s5, iter);
+  if (k == 0) control_update(b, p, s5, iter);
 }
 
 // --------------------------------------------------------------------------
@@ -469,7 +493,12 @@ hipError_t launch_row(const Geom& g, const FftPlan& rp, const Buffers& b,
 
 hipError_t launch_reduce(const Buffers& b, const Params& p, int nblocks,
                          int iter, int /*max_iter*/, hipStream_t st) {
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, b, p, nblocks,
+  const int nblk = nblocks < kReduceBlocks ? 1 : kReduceBlocks;
+  hipLaunchKernelGGL(k_reduce_blocks, dim3(nblk), dim3(kReduceThreads), 0, st,
+                     b, nblocks);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, st, b, p, nblk,
                      iter);
   return hipGetLastError();
 }

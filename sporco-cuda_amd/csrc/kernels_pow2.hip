@@ -13,6 +13,8 @@
 //                 / dual update / residual sums with Y and U prefetched as
 //                 float4 -> Y - U -> forward real FFT -> spectrum.
 
+#include <cstdlib>
+
 #include "fft_reg.h"
 #include "scc_device.h"
 #include "scc_pow2.h"
@@ -21,7 +23,6 @@ namespace scc {
 
 namespace {
 
-constexpr int kRowE = 8;  // points per thread in the row transforms
 constexpr int kColE = 16;  // points per thread in the column transforms
 
 // One element of the fused update on a value x (= X, already / N).
@@ -49,25 +50,6 @@ __device__ __forceinline__ void upd1(const Params& p, float x, float yo,
   acc.s += dy * dy;
 }
 
-// Global address of element i (panel order) of the NR-row block starting at
-// row r0 of map base `mb`, and the (row, column) it holds.
-template <int LR, int NR, int TC>
-__device__ __forceinline__ long long row_spec_addr(int i, int R, int r0,
-                                                   int& rr, int& c) {
-  constexpr int P0 = LR / TC;  // full panels; the last one is column LR
-  constexpr int PER = NR * TC;
-  if (i < P0 * PER) {
-    const int pp = i / PER;
-    const int rem = i - pp * PER;
-    rr = rem / TC;
-    c = pp * TC + (rem - rr * TC);
-    return (long long)pp * TC * R + (long long)r0 * TC + rem;
-  }
-  rr = i - P0 * PER;
-  c = LR;
-  return (long long)P0 * TC * R + r0 + rr;
-}
-
 }  // namespace
 
 // --------------------------------------------------------------------------
@@ -79,7 +61,11 @@ __global__ __launch_bounds__(TC*(R / kColE)) void k_col_pow2(
     const Ctrl* ctrl) {
   constexpr int E = kColE;
   constexpr int T = R / E;
-  constexpr int LP = reg::lds_len(R);
+  // Column stride congruent to 32 / TC (mod 32) in 8-byte words: every
+  // stage's exchange then hits each LDS bank pair exactly twice per
+  // wavefront access (the minimum for 64 lanes of 8 bytes).
+  constexpr int L0 = reg::lds_len(R);
+  constexpr int LP = L0 + ((32 / TC - L0 % 32) + 32) % 32;
   __shared__ float2 lds[TC * LP];
   if (ctrl != nullptr && ctrl->done) return;
   const int p = blockIdx.x;
@@ -98,20 +84,17 @@ __global__ __launch_bounds__(TC*(R / kColE)) void k_col_pow2(
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = make_float2(0.f, 0.f);
   }
-  reg::fft_dev<R, E, DIR>(v, t, L, twR, 1);
+  reg::fft_block<R, E, DIR>(v, t, L, twR, 1);
   if (on) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int r = t + T * e;
-      base[r * w + c] = L[reg::lpad(r)];
-    }
+    for (int e = 0; e < E; ++e) base[(t + T * e) * w + c] = v[e];
   }
 }
 
 // --------------------------------------------------------------------------
 // Sherman-Morrison X-solve: G lanes per frequency, MPT maps per lane.
 // --------------------------------------------------------------------------
-template <int G, int MPT>
+template <int G, int MPT, bool GRAD>
 __global__ __launch_bounds__(256) void k_solve_pow2(Geom g, Buffers b,
                                                     Params p) {
   if (b.ctrl->done) return;
@@ -124,14 +107,18 @@ __global__ __launch_bounds__(256) void k_solve_pow2(Geom g, Buffers b,
       ((long long)blockIdx.x * (blockDim.x >> 6) + wv) * FPW + fl;
   const bool valid = f < g.Hc;
   const float rho = b.ctrl->rho;
+  const float irho = 1.f / rho;
   const int M = g.M;
   float2 sh = make_float2(0.f, 0.f);
   float gh = 0.f;
   if (valid) {
     sh = b.Sf[f];
-    if (p.grad) gh = p.mu * b.ghg[f];
+    if (GRAD) gh = p.mu * b.ghg[f];
   }
+  // b_m and D_m stay in registers between the two passes; 1/delta_m too
+  // when it varies with the map (GradReg), so no division is repeated.
   float2 bb[MPT], dd[MPT];
+  float id[GRAD ? MPT : 1];
   float2 num = make_float2(0.f, 0.f);
   float den = 0.f;
 #pragma unroll
@@ -142,11 +129,11 @@ __global__ __launch_bounds__(256) void k_solve_pow2(Geom g, Buffers b,
       const float2 d = b.Df[o];
       const float2 a = b.F[o];
       const float2 x = cadd(cmul(cconj(d), sh), cscale(a, rho));
-      float inv;
-      if (p.grad)
+      float inv = irho;
+      if (GRAD) {
         inv = 1.f / ((p.gw_mode ? b.gw[m] : p.gw_scalar) * gh + rho);
-      else
-        inv = 1.f / rho;
+        id[i] = inv;
+      }
       num = cadd(num, cscale(cmul(d, x), inv));
       den += (d.x * d.x + d.y * d.y) * inv;
       bb[i] = x;
@@ -164,207 +151,334 @@ __global__ __launch_bounds__(256) void k_solve_pow2(Geom g, Buffers b,
   for (int i = 0; i < MPT; ++i) {
     const int m = gi * MPT + i;
     if (valid && m < M) {
-      float delta = rho;
-      if (p.grad) delta = (p.gw_mode ? b.gw[m] : p.gw_scalar) * gh + rho;
-      const float2 x = csub(bb[i], cmul(cconj(dd[i]), cb));
-      b.F[(long long)m * g.Hc + f] = make_float2(x.x / delta, x.y / delta);
+      const float inv = GRAD ? id[i] : irho;
+      b.F[(long long)m * g.Hc + f] =
+          cscale(csub(bb[i], cmul(cconj(dd[i]), cb)), inv);
+    }
+  }
+}
+
+// Variant with the map groups spread over the four wavefronts of a block:
+// every wavefront covers 64 consecutive frequencies (512 contiguous bytes
+// per load instruction) for maps [w MPT, (w + 1) MPT); the four partial sums
+// are combined through LDS in a fixed order.
+template <int MPT, bool GRAD>
+__global__ __launch_bounds__(256) void k_solve_pow2_wg(Geom g, Buffers b,
+                                                       Params p) {
+  __shared__ float3 part[4][64];
+  if (b.ctrl->done) return;
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const long long f = (long long)blockIdx.x * 64 + lane;
+  const bool valid = f < g.Hc;
+  const float rho = b.ctrl->rho;
+  const float irho = 1.f / rho;
+  const int M = g.M;
+  float2 sh = make_float2(0.f, 0.f);
+  float gh = 0.f;
+  if (valid) {
+    sh = b.Sf[f];
+    if (GRAD) gh = p.mu * b.ghg[f];
+  }
+  float2 bb[MPT], dd[MPT];
+  float id[GRAD ? MPT : 1];
+  float2 num = make_float2(0.f, 0.f);
+  float den = 0.f;
+#pragma unroll
+  for (int i = 0; i < MPT; ++i) {
+    const int m = wv * MPT + i;
+    if (valid && m < M) {
+      const long long o = (long long)m * g.Hc + f;
+      const float2 d = b.Df[o];
+      const float2 a = b.F[o];
+      const float2 x = cadd(cmul(cconj(d), sh), cscale(a, rho));
+      float inv = irho;
+      if (GRAD) {
+        inv = 1.f / ((p.gw_mode ? b.gw[m] : p.gw_scalar) * gh + rho);
+        id[i] = inv;
+      }
+      num = cadd(num, cscale(cmul(d, x), inv));
+      den += (d.x * d.x + d.y * d.y) * inv;
+      bb[i] = x;
+      dd[i] = d;
+    }
+  }
+  part[wv][lane] = make_float3(num.x, num.y, den);
+  __syncthreads();
+  const float3 p0 = part[0][lane], p1 = part[1][lane], p2 = part[2][lane],
+               p3 = part[3][lane];
+  num = make_float2((p0.x + p1.x) + (p2.x + p3.x),
+                    (p0.y + p1.y) + (p2.y + p3.y));
+  den = (p0.z + p1.z) + (p2.z + p3.z);
+  const float2 cb = cscale(num, 1.f / (1.f + den));
+#pragma unroll
+  for (int i = 0; i < MPT; ++i) {
+    const int m = wv * MPT + i;
+    if (valid && m < M) {
+      const float inv = GRAD ? id[i] : irho;
+      b.F[(long long)m * g.Hc + f] =
+          cscale(csub(bb[i], cmul(cconj(dd[i]), cb)), inv);
     }
   }
 }
 
 // --------------------------------------------------------------------------
-// Row kernel
+// Row kernels: every row belongs to one wavefront (RPW rows per wavefront,
+// T lanes per row, E points per lane), so a row never waits on a block
+// barrier.  The half spectrum is read and written in slot order (lane t
+// holds columns t + T e); the real-FFT pre/post-processing pairs column k
+// with column L - k, which lives in lane (T - t) mod T, slot E - 1 - e
+// (lane 0: its own slot E - e, or column L), fetched with a lane shuffle.
+// The last FFT stage leaves its outputs in slot order, which is also the
+// order of the real row as float2 pairs, so the update and the forward
+// transform of Y - U need no LDS pass at all.
 // --------------------------------------------------------------------------
-template <int LR, int NR, int TC>
-__global__ __launch_bounds__(NR*(LR / kRowE)) void k_row_pow2(Geom g,
-                                                              Buffers b,
-                                                              Params p,
-                                                              int mode) {
-  constexpr int E = kRowE;
-  constexpr int T = LR / E;
-  constexpr int NT = NR * T;
-  constexpr int C = 2 * LR;
-  constexpr int H = LR + 1;
-  constexpr int LP = reg::lds_len(LR);
-  constexpr int NW = NT / 64;
-  constexpr int NF4 = E / 2;  // float4 of Y / U per thread
-  __shared__ float2 lds[NR * LP];
-  __shared__ float red[5][NW];
-  if (b.ctrl->done) return;
-  const int tid = threadIdx.x;
-  const int rl = tid / T;
-  const int t = tid - rl * T;
-  const int R = g.R;
-  const long long grow0 = (long long)blockIdx.x * NR;
-  const int m = (int)(grow0 / R);  // rows of a block share the map
-  const int r0 = (int)(grow0 - (long long)m * R);
-  const int r = r0 + rl;
-  const long long rowoff = (grow0 + rl) * C;
-  float2* L = lds + rl * LP;
-  float2* Fm = b.F + (long long)m * g.Hc;
+template <int LR>
+struct RowCfg {
+  static constexpr int E = LR <= 512 ? 8 : 16;
+  static constexpr int T = LR / E;       // lanes per row
+  static constexpr int RPW = 64 / T;     // rows per wavefront
+  static constexpr int RPB = 4 * RPW;    // rows per 256-thread block
+  // Y and U are loaded with the spectrum (latency hidden behind the inverse
+  // transform) where registers allow it.
+  static constexpr bool PREFETCH = E <= 8;
+  // Occupancy target for the register allocator (4 waves per SIMD fits the
+  // E = 8 plain update without spills).
+  static constexpr int WAVES = E <= 8 ? 4 : 2;
+};
 
-  // Prefetch Y and U of this thread's float4 columns.
-  float4 y4[NF4], u4[NF4];
-  const float4* Yr = reinterpret_cast<const float4*>(b.Y + rowoff);
-  const float4* Ur = reinterpret_cast<const float4*>(b.U + rowoff);
-#pragma unroll
-  for (int i = 0; i < NF4; ++i) {
-    y4[i] = Yr[t + T * i];
-    u4[i] = Ur[t + T * i];
-  }
-  // 1. Half spectra of the NR rows into LDS.
-  for (int i = tid; i < NR * H; i += NT) {
-    int rr, c;
-    const long long a = row_spec_addr<LR, NR, TC>(i, R, r0, rr, c);
-    lds[rr * LP + reg::lpad(c)] = Fm[a];
-  }
-  __syncthreads();
-  // 2. Inverse real-FFT pre-processing into this thread's slots:
-  //    Z'[k] = (X[k] + conj X[L-k]) + i W_C^-k (X[k] - conj X[L-k]).
-  float2 v[E];
+__device__ __forceinline__ float2 shfl2(float2 v, int src) {
+  return make_float2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
+}
+
+// Offset of half-spectrum column k of row r in a map (panel layout with
+// full panels of TC columns and a last panel holding column LR).
+template <int TC>
+__device__ __forceinline__ int spec_off(int k, int r, int R) {
+  return (k / TC) * (TC * R) + r * TC + (k % TC);
+}
+
+// X[k] from the forward transform Z of the packed real row:
+// X[k] = (Z[k] + conj Z[L-k]) / 2 - i W_C^k (Z[k] - conj Z[L-k]) / 2.
+__device__ __forceinline__ float2 r2c_post(float2 zk, float2 znc, float2 w) {
+  const float2 s = cadd(zk, znc);
+  const float2 wd = cmul(w, csub(zk, znc));
+  return make_float2(0.5f * (s.x + wd.y), 0.5f * (s.y - wd.x));
+}
+
+template <int LR, int TC>
+__device__ __forceinline__ void store_spectrum(const float2 (&z)[RowCfg<LR>::E],
+                                               int t, int partner, int r,
+                                               int R, float2* Fm,
+                                               const float2* twC) {
+  using RC = RowCfg<LR>;
+  constexpr int E = RC::E, T = RC::T;
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int k = t + T * e;
-    float2 xk = L[reg::lpad(k)];
-    float2 xn = L[reg::lpad(LR - k)];
-    if (k == 0) {
+    const float2 sh = shfl2(z[E - 1 - e], partner);
+    const float2 zn = (t == 0) ? (e == 0 ? z[0] : z[(E - e) % E]) : sh;
+    Fm[spec_off<TC>(k, r, R)] = r2c_post(z[e], cconj(zn), twC[k]);
+  }
+  if (t == 0)  // column L: Re Z[0] - Im Z[0]
+    Fm[(LR / TC) * TC * R + r] = make_float2(z[0].x - z[0].y, 0.f);
+}
+
+// GENERAL selects the update with an AMS mask, a full weight array,
+// NonNegCoef or NoBndryCross; without it the threshold is one value per row.
+template <int LR, int TC, bool GENERAL>
+__global__ __launch_bounds__(256) __attribute__((
+    amdgpu_waves_per_eu(GENERAL ? 1 : RowCfg<LR>::WAVES))) void
+k_row_pow2(Geom g, Buffers b, Params p, int mode) {
+  using RC = RowCfg<LR>;
+  constexpr int E = RC::E, T = RC::T, RPW = RC::RPW;
+  constexpr int C = 2 * LR;
+  constexpr int LP = reg::lds_len(LR);
+  __shared__ float2 lds[4 * RPW * LP];
+  if (b.ctrl->done) return;
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int rw = lane / T;
+  const int t = lane & (T - 1);
+  const int partner = rw * T + ((T - t) & (T - 1));
+  // Wave-uniform bases (the RPW rows of a wavefront share one map) keep
+  // every per-lane address a 32-bit offset from a scalar base.
+  const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wv);
+  const int row0 = wave * RPW;
+  const int R = g.R;
+  const int m = row0 >> g.lgR;
+  const int r = (row0 & (R - 1)) + rw;
+  const int row = row0 + rw;
+  float2* L = lds + (wv * RPW + rw) * LP;
+  float2* Fm = b.F + (long long)m * g.Hc;
+  const long long rowoff = (long long)row * C;
+  float2* Y2 = reinterpret_cast<float2*>(b.Y + (long long)row0 * C);
+  float2* U2 = reinterpret_cast<float2*>(b.U + (long long)row0 * C);
+  const int lo = rw * LR + t;  // this lane's first float2 of its row
+
+  // Half spectrum in slot order, column L on lane 0; Y and U prefetched.
+  float2 v[E], y2[E], u2[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = Fm[spec_off<TC>(t + T * e, r, R)];
+  float2 xL = make_float2(0.f, 0.f);
+  if (t == 0) xL = Fm[(LR / TC) * TC * R + r];
+  if (RC::PREFETCH) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      y2[e] = Y2[lo + T * e];
+      u2[e] = U2[lo + T * e];
+    }
+  }
+  // Inverse real-FFT pre-processing:
+  // Z'[k] = (X[k] + conj X[L-k]) + i W_C^-k (X[k] - conj X[L-k]).
+  float2 z[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = t + T * e;
+    const float2 sh = shfl2(v[E - 1 - e], partner);
+    float2 xn = (t == 0) ? (e == 0 ? xL : v[(E - e) % E]) : sh;
+    float2 xk = v[e];
+    if (t == 0 && e == 0) {  // DC and Nyquist are real
       xk.y = 0.f;
       xn.y = 0.f;
     }
     xn.y = -xn.y;
     const float2 s = cadd(xk, xn);
     const float2 wd = cmul(cconj(b.twC[k]), csub(xk, xn));
-    v[e] = make_float2(s.x - wd.y, s.y + wd.x);
+    z[e] = make_float2(s.x - wd.y, s.y + wd.x);
   }
-  __syncthreads();
-  reg::fft_dev<LR, E, -1>(v, t, L, b.twC, 2);
+  reg::fft_wave<LR, E, -1>(z, t, L, b.twC, 2);
 
-  // 4. Fused update on X = z / N (z interleaves x[2n], x[2n+1]).
+  // Fused update on the real row: slot e holds x[2k], x[2k+1] (times N).
   const float rho = b.ctrl->rho;
   const float invN = 1.0f / (float)g.N;
   const float lr = p.lambda / rho;
-  const bool ams = m < p.m0;
+  const bool ams = GENERAL && m < p.m0;
   const float wmap = p.l1_mode == 0 ? p.l1_scalar
                      : (p.l1_mode == 1 && !ams ? b.l1w[m] : 0.f);
   const bool bndry_row = p.nobndry && r > R - p.Kh;
   UpdateSums acc;
-  float4* Yw = reinterpret_cast<float4*>(b.Y + rowoff);
-  float4* Uw = reinterpret_cast<float4*>(b.U + rowoff);
+  if (!RC::PREFETCH) {
 #pragma unroll
-  for (int i = 0; i < NF4; ++i) {
-    const int f4 = t + T * i;
-    const int n0 = 4 * f4;
-    const float2 z0 = L[reg::lpad(2 * f4)];
-    const float2 z1 = L[reg::lpad(2 * f4 + 1)];
-    const float xs[4] = {z0.x * invN, z0.y * invN, z1.x * invN, z1.y * invN};
-    const float ys[4] = {y4[i].x, y4[i].y, y4[i].z, y4[i].w};
-    const float us[4] = {u4[i].x, u4[i].y, u4[i].z, u4[i].w};
-    float th[4] = {lr * wmap, lr * wmap, lr * wmap, lr * wmap};
-    int mw[4] = {0, 0, 0, 0};
-    if (ams) {
-      const int4 w4 = reinterpret_cast<const int4*>(b.mask + (long long)r * C)[f4];
-      mw[0] = w4.x;
-      mw[1] = w4.y;
-      mw[2] = w4.z;
-      mw[3] = w4.w;
-    } else if (p.l1_mode == 2) {
-      const float4 w4 = reinterpret_cast<const float4*>(b.l1w + rowoff)[f4];
-      th[0] = lr * w4.x;
-      th[1] = lr * w4.y;
-      th[2] = lr * w4.z;
-      th[3] = lr * w4.w;
+    for (int e = 0; e < E; ++e) {
+      y2[e] = Y2[lo + T * e];
+      u2[e] = U2[lo + T * e];
     }
-    float yn[4], un[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const bool zb = p.nobndry && (bndry_row || n0 + j > C - p.Kw);
-      upd1(p, xs[j], ys[j], us[j], th[j], mw[j], ams, zb, yn[j], un[j], acc);
-    }
-    Yw[f4] = make_float4(yn[0], yn[1], yn[2], yn[3]);
-    Uw[f4] = make_float4(un[0], un[1], un[2], un[3]);
-    L[reg::lpad(2 * f4)] = make_float2(yn[0] - un[0], yn[1] - un[1]);
-    L[reg::lpad(2 * f4 + 1)] = make_float2(yn[2] - un[2], yn[3] - un[3]);
   }
-  block_sums_store<NW>(acc, red, b.partials, gridDim.x, blockIdx.x, tid);
+  if (GENERAL) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = t + T * e;
+      const int n0 = 2 * k;
+      float th0 = lr * wmap, th1 = lr * wmap;
+      int w0 = 0, w1 = 0;
+      if (ams) {
+        const int2 w2 =
+            reinterpret_cast<const int2*>(b.mask + (long long)r * C)[k];
+        w0 = w2.x;
+        w1 = w2.y;
+      } else if (p.l1_mode == 2) {
+        const float2 w2 = reinterpret_cast<const float2*>(b.l1w + rowoff)[k];
+        th0 = lr * w2.x;
+        th1 = lr * w2.y;
+      }
+      const bool zb0 = p.nobndry && (bndry_row || n0 > C - p.Kw);
+      const bool zb1 = p.nobndry && (bndry_row || n0 + 1 > C - p.Kw);
+      float yn0, un0, yn1, un1;
+      upd1(p, z[e].x * invN, y2[e].x, u2[e].x, th0, w0, ams, zb0, yn0, un0,
+           acc);
+      upd1(p, z[e].y * invN, y2[e].y, u2[e].y, th1, w1, ams, zb1, yn1, un1,
+           acc);
+      Y2[lo + T * e] = make_float2(yn0, yn1);
+      U2[lo + T * e] = make_float2(un0, un1);
+      z[e] = make_float2(yn0 - un0, yn1 - un1);
+    }
+  } else {
+    // Plain update: Xr = a X + (1 - a) Yold, V = Xr + U,
+    // Y = copysign(max(|V| - t, 0), V), U = V - Y.
+    const float th = lr * wmap;
+    const float a = p.alpha, a1 = 1.f - p.alpha;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const float x0 = z[e].x * invN, x1 = z[e].y * invN;
+      const float v0 = fmaf(a, x0, fmaf(a1, y2[e].x, u2[e].x));
+      const float v1 = fmaf(a, x1, fmaf(a1, y2[e].y, u2[e].y));
+      const float yn0 = copysignf(fmaxf(fabsf(v0) - th, 0.f), v0);
+      const float yn1 = copysignf(fmaxf(fabsf(v1) - th, 0.f), v1);
+      const float un0 = v0 - yn0, un1 = v1 - yn1;
+      acc.x = fmaf(x0, x0, fmaf(x1, x1, acc.x));
+      acc.y = fmaf(yn0, yn0, fmaf(yn1, yn1, acc.y));
+      acc.u = fmaf(un0, un0, fmaf(un1, un1, acc.u));
+      const float d0 = x0 - yn0, d1 = x1 - yn1;
+      acc.r = fmaf(d0, d0, fmaf(d1, d1, acc.r));
+      const float s0 = y2[e].x - yn0, s1 = y2[e].y - yn1;
+      acc.s = fmaf(s0, s0, fmaf(s1, s1, acc.s));
+      Y2[lo + T * e] = make_float2(yn0, yn1);
+      U2[lo + T * e] = make_float2(un0, un1);
+      z[e] = make_float2(yn0 - un0, yn1 - un1);
+    }
+  }
+  // Per-wavefront partial sums (no block barrier).
+  {
+    const float sv[5] = {acc.x, acc.y, acc.u, acc.r, acc.s};
+    const int nwaves = gridDim.x * 4;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      const float s = wave_sum(sv[j]);
+      if (lane == 0) b.partials[j * nwaves + wave] = s;
+    }
+  }
   if (mode != kRowFused) return;
-  // 5. Forward transform of Y - U (the block barrier of block_sums_store
-  //    orders the LDS writes above before these reads).
-  reg::slots_read<LR, E>(v, t, L);
-  __syncthreads();
-  reg::fft_dev<LR, E, 1>(v, t, L, b.twC, 2);
-  // 6. Real-FFT post-processing and store in panel order:
-  //    X[k] = (Z[k] + conj Z[L-k]) / 2 - i W_C^k (Z[k] - conj Z[L-k]) / 2.
-  for (int i = tid; i < NR * H; i += NT) {
-    int rr, c;
-    const long long a = row_spec_addr<LR, NR, TC>(i, R, r0, rr, c);
-    const float2* Z = lds + rr * LP;
-    const float2 zk = Z[reg::lpad(c == LR ? 0 : c)];
-    const float2 zn = cconj(Z[reg::lpad(c == 0 ? 0 : LR - c)]);
-    const float2 s = cadd(zk, zn);
-    const float2 wd = cmul(b.twC[c], csub(zk, zn));
-    Fm[a] = make_float2(0.5f * (s.x + wd.y), 0.5f * (s.y - wd.x));
-  }
+  reg::fft_wave<LR, E, 1>(z, t, L, b.twC, 2);
+  store_spectrum<LR, TC>(z, t, partner, r, R, Fm, b.twC);
 }
 
 // Forward row transform of a real array (setup) or of Y - U / rsf.
-template <int LR, int NR, int TC>
-__global__ __launch_bounds__(NR*(LR / kRowE)) void k_row_fwd_pow2(
+template <int LR, int TC>
+__global__ __launch_bounds__(256) void k_row_fwd_pow2(
     Geom g, int mode, const float* __restrict__ src, float* Y, float* U,
     float2* __restrict__ F, const float2* __restrict__ twC,
     const Ctrl* ctrl) {
-  constexpr int E = kRowE;
-  constexpr int T = LR / E;
-  constexpr int NT = NR * T;
+  using RC = RowCfg<LR>;
+  constexpr int E = RC::E, T = RC::T, RPW = RC::RPW;
   constexpr int C = 2 * LR;
-  constexpr int H = LR + 1;
   constexpr int LP = reg::lds_len(LR);
-  constexpr int NF4 = E / 2;
-  __shared__ float2 lds[NR * LP];
+  __shared__ float2 lds[4 * RPW * LP];
   if (mode == kFwdYU && ctrl->done) return;
-  const int tid = threadIdx.x;
-  const int rl = tid / T;
-  const int t = tid - rl * T;
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int rw = lane / T;
+  const int t = lane & (T - 1);
+  const int partner = rw * T + ((T - t) & (T - 1));
+  const int row0 = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wv) * RPW;
   const int R = g.R;
-  const long long grow0 = (long long)blockIdx.x * NR;
-  const int m = (int)(grow0 / R);
-  const int r0 = (int)(grow0 - (long long)m * R);
-  const long long rowoff = (grow0 + rl) * C;
-  float2* L = lds + rl * LP;
-  const float rsf = (mode == kFwdYU) ? ctrl->rsf : 1.f;
+  const int m = row0 >> g.lgR;
+  const int r = (row0 & (R - 1)) + rw;
+  float2* L = lds + (wv * RPW + rw) * LP;
+  const long long rowoff = (long long)row0 * C;
+  const int lo = rw * LR + t;
+  float2 z[E];
+  if (mode == kFwdReal) {
+    const float2* S2 = reinterpret_cast<const float2*>(src + rowoff);
 #pragma unroll
-  for (int i = 0; i < NF4; ++i) {
-    const int f4 = t + T * i;
-    float4 a;
-    if (mode == kFwdReal) {
-      a = reinterpret_cast<const float4*>(src + rowoff)[f4];
-    } else {
-      const float4 y = reinterpret_cast<const float4*>(Y + rowoff)[f4];
-      float4 u = reinterpret_cast<const float4*>(U + rowoff)[f4];
+    for (int e = 0; e < E; ++e) z[e] = S2[lo + T * e];
+  } else {
+    const float rsf = ctrl->rsf;
+    const float2* Y2 = reinterpret_cast<const float2*>(Y + rowoff);
+    float2* U2 = reinterpret_cast<float2*>(U + rowoff);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const float2 y = Y2[lo + T * e];
+      float2 u = U2[lo + T * e];
       if (rsf != 1.f) {
-        u = make_float4(u.x / rsf, u.y / rsf, u.z / rsf, u.w / rsf);
-        reinterpret_cast<float4*>(U + rowoff)[f4] = u;
+        u = make_float2(u.x / rsf, u.y / rsf);
+        U2[lo + T * e] = u;
       }
-      a = make_float4(y.x - u.x, y.y - u.y, y.z - u.z, y.w - u.w);
+      z[e] = make_float2(y.x - u.x, y.y - u.y);
     }
-    L[reg::lpad(2 * f4)] = make_float2(a.x, a.y);
-    L[reg::lpad(2 * f4 + 1)] = make_float2(a.z, a.w);
   }
-  __syncthreads();
-  float2 v[E];
-  reg::slots_read<LR, E>(v, t, L);
-  __syncthreads();
-  reg::fft_dev<LR, E, 1>(v, t, L, twC, 2);
-  float2* Fm = F + (long long)m * g.Hc;
-  for (int i = tid; i < NR * H; i += NT) {
-    int rr, c;
-    const long long a = row_spec_addr<LR, NR, TC>(i, R, r0, rr, c);
-    const float2* Z = lds + rr * LP;
-    const float2 zk = Z[reg::lpad(c == LR ? 0 : c)];
-    const float2 zn = cconj(Z[reg::lpad(c == 0 ? 0 : LR - c)]);
-    const float2 s = cadd(zk, zn);
-    const float2 wd = cmul(twC[c], csub(zk, zn));
-    Fm[a] = make_float2(0.5f * (s.x + wd.y), 0.5f * (s.y - wd.x));
-  }
+  reg::fft_wave<LR, E, 1>(z, t, L, twC, 2);
+  store_spectrum<LR, TC>(z, t, partner, r, R, F + (long long)m * g.Hc, twC);
 }
 
 // --------------------------------------------------------------------------
@@ -377,36 +491,36 @@ bool is_pow2(int x) { return x > 0 && (x & (x - 1)) == 0; }
 int col_tc(int R) {
   return R <= 512 ? 16 : (R == 1024 ? 8 : 4);
 }
-// Rows per row-kernel block: 256 threads of kRowE points each.
-int row_nr(int C) { return 256 / ((C / 2) / kRowE); }
 
-template <int LR, int NR, int TC>
+template <int LR, int TC>
 hipError_t row_launch(const Geom& g, const Buffers& b, const Params& p,
                       int mode, int* nblocks, hipStream_t st) {
-  const unsigned nb = (unsigned)((long long)g.M * g.R / NR);
-  hipLaunchKernelGGL((k_row_pow2<LR, NR, TC>), dim3(nb), dim3(NR * LR / kRowE),
-                     0, st, g, b, p, mode);
-  *nblocks = (int)nb;
+  const unsigned nb = (unsigned)((long long)g.M * g.R / RowCfg<LR>::RPB);
+  const bool general = p.m0 || p.l1_mode == 2 || p.nonneg || p.nobndry;
+  if (general)
+    hipLaunchKernelGGL((k_row_pow2<LR, TC, true>), dim3(nb), dim3(256), 0, st,
+                       g, b, p, mode);
+  else
+    hipLaunchKernelGGL((k_row_pow2<LR, TC, false>), dim3(nb), dim3(256), 0,
+                       st, g, b, p, mode);
+  *nblocks = (int)nb * 4;  // partial sums per wavefront
   return hipGetLastError();
 }
 
-template <int LR, int NR, int TC>
+template <int LR, int TC>
 hipError_t row_fwd_launch(const Geom& g, int nmaps, int mode, const float* src,
                           float* Y, float* U, float2* F, const float2* twC,
                           const Ctrl* ctrl, hipStream_t st) {
-  const unsigned nb = (unsigned)((long long)nmaps * g.R / NR);
-  hipLaunchKernelGGL((k_row_fwd_pow2<LR, NR, TC>), dim3(nb),
-                     dim3(NR * LR / kRowE), 0, st, g, mode, src, Y, U, F, twC,
-                     ctrl);
+  const unsigned nb = (unsigned)((long long)nmaps * g.R / RowCfg<LR>::RPB);
+  hipLaunchKernelGGL((k_row_fwd_pow2<LR, TC>), dim3(nb), dim3(256), 0, st, g,
+                     mode, src, Y, U, F, twC, ctrl);
   return hipGetLastError();
 }
 
-// (C, TC) combinations: C in 256..2048, TC from R.
-#define SCC_ROW_CASES(X)              \
-  X(128, 16, 16) X(128, 16, 8) X(128, 16, 4) \
-  X(256, 8, 16) X(256, 8, 8) X(256, 8, 4)    \
-  X(512, 4, 16) X(512, 4, 8) X(512, 4, 4)    \
-  X(1024, 2, 16) X(1024, 2, 8) X(1024, 2, 4)
+// (C/2, TC) combinations: C in 256..2048, TC from R.
+#define SCC_ROW_CASES(X)                                      \
+  X(128, 16) X(128, 8) X(128, 4) X(256, 16) X(256, 8) X(256, 4) \
+  X(512, 16) X(512, 8) X(512, 4) X(1024, 16) X(1024, 8) X(1024, 4)
 
 }  // namespace
 
@@ -415,7 +529,7 @@ bool pow2_select(int R, int C, int M, Pow2Config* cfg) {
   if (R < 256 || R > 2048 || C < 256 || C > 2048) return false;
   if (M < 1 || M > 256) return false;
   cfg->TC = col_tc(R);
-  cfg->rows = row_nr(C);
+  cfg->rows = C / 2 <= 512 ? 64 / ((C / 2) / 8) : 1;  // rows per wavefront
   int G = 1;
   while (G * 16 < M) G *= 2;
   cfg->variant = G;
@@ -423,15 +537,15 @@ bool pow2_select(int R, int C, int M, Pow2Config* cfg) {
 }
 
 int pow2_row_blocks(const Geom& g, const Pow2Config& cfg) {
-  return (int)((long long)g.M * g.R / cfg.rows);
+  return (int)((long long)g.M * g.R / cfg.rows);  // one partial per wavefront
 }
 
 hipError_t pow2_row(const Geom& g, const Pow2Config& cfg, const Buffers& b,
                     const Params& p, int mode, int* nblocks, hipStream_t st) {
   const int LR = g.C / 2;
-#define X(lr, nr, tc)                                               \
+#define X(lr, tc)                                                   \
   if (LR == lr && cfg.TC == tc)                                     \
-    return row_launch<lr, nr, tc>(g, b, p, mode, nblocks, st);
+    return row_launch<lr, tc>(g, b, p, mode, nblocks, st);
   SCC_ROW_CASES(X)
 #undef X
   return hipErrorNotSupported;
@@ -442,10 +556,10 @@ hipError_t pow2_row_fwd(const Geom& g, const Pow2Config& cfg, int nmaps,
                         float2* F, const float2* twC, const Ctrl* ctrl,
                         hipStream_t st) {
   const int LR = g.C / 2;
-#define X(lr, nr, tc)                                                       \
+#define X(lr, tc)                                                           \
   if (LR == lr && cfg.TC == tc)                                             \
-    return row_fwd_launch<lr, nr, tc>(g, nmaps, mode, src, Y, U, F, twC,    \
-                                      ctrl, st);
+    return row_fwd_launch<lr, tc>(g, nmaps, mode, src, Y, U, F, twC, ctrl,  \
+                                  st);
   SCC_ROW_CASES(X)
 #undef X
   return hipErrorNotSupported;
@@ -474,13 +588,31 @@ hipError_t pow2_col(const Geom& g, const Pow2Config& cfg, int nmaps,
 hipError_t pow2_solve(const Geom& g, const Pow2Config& cfg, const Buffers& b,
                       const Params& p, hipStream_t st) {
   const int G = cfg.variant;
+  static const int wg_variant = [] {
+    const char* v = getenv("SCC_SOLVE_WG");
+    return v ? atoi(v) : 0;
+  }();
+  if (wg_variant && g.M <= 64) {
+    const unsigned nb = (unsigned)((g.Hc + 63) / 64);
+    if (p.grad)
+      hipLaunchKernelGGL((k_solve_pow2_wg<16, true>), dim3(nb), dim3(256), 0,
+                         st, g, b, p);
+    else
+      hipLaunchKernelGGL((k_solve_pow2_wg<16, false>), dim3(nb), dim3(256), 0,
+                         st, g, b, p);
+    return hipGetLastError();
+  }
   const long long fpb = 4LL * (64 / G);  // frequencies per 256-thread block
   const unsigned nb = (unsigned)((g.Hc + fpb - 1) / fpb);
-#define X(gg)                                                                \
-  if (G == gg) {                                                             \
-    hipLaunchKernelGGL((k_solve_pow2<gg, 16>), dim3(nb), dim3(256), 0, st, g, \
-                       b, p);                                                \
-    return hipGetLastError();                                                \
+#define X(gg)                                                              \
+  if (G == gg) {                                                           \
+    if (p.grad)                                                            \
+      hipLaunchKernelGGL((k_solve_pow2<gg, 16, true>), dim3(nb), dim3(256), \
+                         0, st, g, b, p);                                  \
+    else                                                                   \
+      hipLaunchKernelGGL((k_solve_pow2<gg, 16, false>), dim3(nb),           \
+                         dim3(256), 0, st, g, b, p);                       \
+    return hipGetLastError();                                              \
   }
   X(1) X(2) X(4) X(8) X(16)
 #undef X

@@ -32,6 +32,7 @@ struct Geom {
   int wlast;                // width of the last panel
   long long N;              // R*C
   long long Hc;             // R*H
+  int lgR;                  // log2(R) when R is a power of two, else -1
 };
 
 // Offset of (r, c) of the half spectrum inside one map's panel layout.
@@ -96,6 +97,7 @@ struct Buffers {
   const float* gw;  // per map gradient weights (M)
   const int* mask;  // (R,C) AMS mask
   float* partials;  // 5 x nblocks residual partial sums
+  double* blocksum; // 5 per reduction block (first reduction pass)
   Ctrl* ctrl;
   float* hist;      // 3 x max_iter history (r, s, rho), optional
   float2* twR;      // W_R^t, t < R

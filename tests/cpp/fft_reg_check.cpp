@@ -33,6 +33,29 @@ void run_stages(std::vector<std::vector<float2>>& v, float2* lds,
   }
 }
 
+// Same stages with the last one kept in registers (fft_wave / fft_block).
+template <int L, int E, int DIR, int S>
+void run_stages_slots(std::vector<std::vector<float2>>& v, float2* lds,
+                      const float2* tw, int tws) {
+  constexpr int T = L / E;
+  for (int t = 0; t < T; ++t) {
+    float2(&r)[E] = *reinterpret_cast<float2(*)[E]>(v[t].data());
+    stage_compute<L, E, S, DIR>(r, t, tw, tws);
+  }
+  if constexpr (S + 1 < Plan<L, E>::stages()) {
+    for (int t = 0; t < T; ++t) {
+      const float2(&r)[E] =
+          *reinterpret_cast<const float2(*)[E]>(v[t].data());
+      stage_write<L, E, S>(r, t, lds);
+    }
+    for (int t = 0; t < T; ++t) {
+      float2(&r)[E] = *reinterpret_cast<float2(*)[E]>(v[t].data());
+      slots_read<L, E>(r, t, lds);
+    }
+    run_stages_slots<L, E, DIR, S + 1>(v, lds, tw, tws);
+  }
+}
+
 template <int L, int E, int DIR>
 int check(int tws) {
   constexpr int T = L / E;
@@ -52,8 +75,11 @@ int check(int tws) {
   for (int t = 0; t < T; ++t)
     for (int e = 0; e < E; ++e)
       v[t][e] = make_float2((float)xr[t + T * e], (float)xi[t + T * e]);
+  std::vector<std::vector<float2>> v2 = v;
   std::vector<float2> lds(lds_len(L));
   run_stages<L, E, DIR, 0>(v, lds.data(), tw.data(), tws);
+  std::vector<float2> lds2(lds_len(L));
+  run_stages_slots<L, E, DIR, 0>(v2, lds2.data(), tw.data(), tws);
   double err = 0, ref = 0;
   for (int k = 0; k < L; ++k) {
     double sr = 0, si = 0;
@@ -64,6 +90,8 @@ int check(int tws) {
     }
     const float2 g = lds[lpad(k)];
     err = fmax(err, fabs(g.x - sr) + fabs(g.y - si));
+    const float2 g2 = v2[k % T][k / T];
+    err = fmax(err, fabs(g2.x - sr) + fabs(g2.y - si));
     ref = fmax(ref, fabs(sr) + fabs(si));
   }
   const double rel = err / ref;
