@@ -13,8 +13,6 @@
 //                 / dual update / residual sums with Y and U prefetched as
 //                 float4 -> Y - U -> forward real FFT -> spectrum.
 
-#include <cstdlib>
-
 #include "fft_reg.h"
 #include "scc_device.h"
 #include "scc_pow2.h"
@@ -150,70 +148,6 @@ __global__ __launch_bounds__(256) void k_solve_pow2(Geom g, Buffers b,
 #pragma unroll
   for (int i = 0; i < MPT; ++i) {
     const int m = gi * MPT + i;
-    if (valid && m < M) {
-      const float inv = GRAD ? id[i] : irho;
-      b.F[(long long)m * g.Hc + f] =
-          cscale(csub(bb[i], cmul(cconj(dd[i]), cb)), inv);
-    }
-  }
-}
-
-// Variant with the map groups spread over the four wavefronts of a block:
-// every wavefront covers 64 consecutive frequencies (512 contiguous bytes
-// per load instruction) for maps [w MPT, (w + 1) MPT); the four partial sums
-// are combined through LDS in a fixed order.
-template <int MPT, bool GRAD>
-__global__ __launch_bounds__(256) void k_solve_pow2_wg(Geom g, Buffers b,
-                                                       Params p) {
-  __shared__ float3 part[4][64];
-  if (b.ctrl->done) return;
-  const int lane = threadIdx.x & 63;
-  const int wv = threadIdx.x >> 6;
-  const long long f = (long long)blockIdx.x * 64 + lane;
-  const bool valid = f < g.Hc;
-  const float rho = b.ctrl->rho;
-  const float irho = 1.f / rho;
-  const int M = g.M;
-  float2 sh = make_float2(0.f, 0.f);
-  float gh = 0.f;
-  if (valid) {
-    sh = b.Sf[f];
-    if (GRAD) gh = p.mu * b.ghg[f];
-  }
-  float2 bb[MPT], dd[MPT];
-  float id[GRAD ? MPT : 1];
-  float2 num = make_float2(0.f, 0.f);
-  float den = 0.f;
-#pragma unroll
-  for (int i = 0; i < MPT; ++i) {
-    const int m = wv * MPT + i;
-    if (valid && m < M) {
-      const long long o = (long long)m * g.Hc + f;
-      const float2 d = b.Df[o];
-      const float2 a = b.F[o];
-      const float2 x = cadd(cmul(cconj(d), sh), cscale(a, rho));
-      float inv = irho;
-      if (GRAD) {
-        inv = 1.f / ((p.gw_mode ? b.gw[m] : p.gw_scalar) * gh + rho);
-        id[i] = inv;
-      }
-      num = cadd(num, cscale(cmul(d, x), inv));
-      den += (d.x * d.x + d.y * d.y) * inv;
-      bb[i] = x;
-      dd[i] = d;
-    }
-  }
-  part[wv][lane] = make_float3(num.x, num.y, den);
-  __syncthreads();
-  const float3 p0 = part[0][lane], p1 = part[1][lane], p2 = part[2][lane],
-               p3 = part[3][lane];
-  num = make_float2((p0.x + p1.x) + (p2.x + p3.x),
-                    (p0.y + p1.y) + (p2.y + p3.y));
-  den = (p0.z + p1.z) + (p2.z + p3.z);
-  const float2 cb = cscale(num, 1.f / (1.f + den));
-#pragma unroll
-  for (int i = 0; i < MPT; ++i) {
-    const int m = wv * MPT + i;
     if (valid && m < M) {
       const float inv = GRAD ? id[i] : irho;
       b.F[(long long)m * g.Hc + f] =
@@ -588,20 +522,6 @@ hipError_t pow2_col(const Geom& g, const Pow2Config& cfg, int nmaps,
 hipError_t pow2_solve(const Geom& g, const Pow2Config& cfg, const Buffers& b,
                       const Params& p, hipStream_t st) {
   const int G = cfg.variant;
-  static const int wg_variant = [] {
-    const char* v = getenv("SCC_SOLVE_WG");
-    return v ? atoi(v) : 0;
-  }();
-  if (wg_variant && g.M <= 64) {
-    const unsigned nb = (unsigned)((g.Hc + 63) / 64);
-    if (p.grad)
-      hipLaunchKernelGGL((k_solve_pow2_wg<16, true>), dim3(nb), dim3(256), 0,
-                         st, g, b, p);
-    else
-      hipLaunchKernelGGL((k_solve_pow2_wg<16, false>), dim3(nb), dim3(256), 0,
-                         st, g, b, p);
-    return hipGetLastError();
-  }
   const long long fpb = 4LL * (64 / G);  // frequencies per 256-thread block
   const unsigned nb = (unsigned)((g.Hc + fpb - 1) / fpb);
 #define X(gg)                                                              \
