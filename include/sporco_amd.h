@@ -134,6 +134,87 @@ int scc_last_history(float *rsr, int cap);
 void scc_set_stage_timing(int enabled);
 int scc_last_stage_times(double *ms_total, int *launches, int n);
 
+/* Differentiable unrolled plain CBPDN over a batch of B images of one
+ * power-of-two size (sides 256..2048) sharing a dictionary: n_iter ADMM
+ * iterations with fixed rho, relaxation alpha, scalar lambda, L1 weight 1.
+ * All arrays are device pointers: D (M, K, K), S (B, R, C), Y (B, M, R, C),
+ * ckpt of scc_unrolled_ckpt_floats(...) floats ((Y, U) every ckpt_every
+ * iterations).  The backward takes dL/dY and writes dL/dD (M, K, K),
+ * dL/dS (B, R, C) and dL/dlambda (one float, host pointer), recomputing each
+ * checkpoint segment and running the hand-written adjoint kernels.  No
+ * counterpart in the reference (extension of the same iteration,
+ * cbpdn.cu:318-567). */
+size_t scc_unrolled_ckpt_floats(int M, int B, int R, int C, int n_iter,
+                                int ckpt_every);
+int scc_unrolled_forward(const float *D, int M, int K, const float *S, int B,
+                         int R, int C, float lambda, float rho, float alpha,
+                         int n_iter, int ckpt_every, float *Y, float *ckpt,
+                         void *stream);
+int scc_unrolled_backward(const float *D, int M, int K, const float *S, int B,
+                          int R, int C, float lambda, float rho, float alpha,
+                          int n_iter, int ckpt_every, const float *ckpt,
+                          const float *gY, float *gD, float *gS,
+                          float *glambda, void *stream);
+/* The same forward, writing V = alpha X + (1 - alpha) Y + U of every
+ * iteration to V (n_iter * B * M * R * C floats, device): the soft-threshold
+ * supports 1[|V| > lambda / rho] the forward took, for checking the adjoint
+ * on the branch of the piecewise-linear iteration the GPU computed. */
+int scc_unrolled_tape(const float *D, int M, int K, const float *S, int B,
+                      int R, int C, float lambda, float rho, float alpha,
+                      int n_iter, float *V, void *stream);
+const char *scc_unrolled_last_error(void);
+
+/* Batched plain CBPDN: nimg images of any sizes sharing one dictionary D
+ * (M, Kh, Kw), each with its own lambda, solved together.  Every ADMM stage
+ * is one launch over a work list that spans all images, and the iteration
+ * loop is replayed from a captured HIP graph; panel width, rows per row
+ * block, lanes per frequency of the solve and the grid shape are chosen by
+ * timing candidates on the batch itself the first time a batch signature
+ * (device, M, image sizes) is seen.  Per image the result equals scc_cbpdn
+ * on that image (same iteration, residuals, stopping test and AutoRho,
+ * applied per image); options are read from opt as there, except the size
+ * fields, which come from the arguments.  L1Weight may hold 1 or M entries;
+ * masks (nWeight > 1) are rejected.  D, images[i].S (R, C) and images[i].Y
+ * (M, R, C) are device pointers; images and results are host arrays;
+ * images[i].rho > 0 sets that image's initial rho, otherwise opt->rho.  No
+ * counterpart in the reference, whose only batch form is a host loop over
+ * cuda_wrapper_CBPDN (cbpdn.h:10). */
+typedef struct {
+  int R, C;
+  const float *S;
+  float *Y;
+  float lambda;
+  float rho;
+} scc_batch_image;
+
+typedef struct {
+  int iterations;
+  float rho, r, s, epri, edua;
+} scc_batch_result;
+
+typedef struct {
+  int col_lds;       /* LDS budget of a column block (bytes) */
+  int row_lds;       /* LDS budget of a row block (bytes) */
+  int solve_lanes;   /* lanes per frequency in the X-solve */
+  int solve_maps;    /* maps per lane in the X-solve */
+  int persistent;    /* 1: occupancy-sized grids, 0: one block per item */
+  int graph;         /* the iterations ran from a captured graph */
+  double tune_ms;    /* autotuning time of this call (0 when cached) */
+  double setup_ms;   /* spectra of D and the images (device time) */
+  double solve_ms;   /* iterations (device time) */
+} scc_batch_config;
+
+int scc_batched_cbpdn(const float *D, int M, int Kh, int Kw, int nimg,
+                      const scc_batch_image *images, const scc_algopt *opt,
+                      scc_batch_result *results, void *stream);
+/* Tuning policy of later calls: 1 autotune new signatures (default), 0 use
+ * the fixed default configuration. */
+int scc_batched_set_autotune(int on);
+/* Graph policy: 1 capture the iterations in a HIP graph (default), 0 launch
+ * them directly. */
+int scc_batched_set_graph(int on);
+int scc_batched_last_config(scc_batch_config *cfg);
+
 /* Device queries: replace get_device_count, get_current_device,
  * set_current_device, get_memory_info, get_device_name
  * (sporco_cuda/src/utils.cu:146-175, bound at util.pyx:3-7). */

@@ -30,33 +30,13 @@
 
 namespace scc {
 namespace {
-
 thread_local std::string g_err;
-thread_local scc_info g_info;
-thread_local bool g_hist_on = false;
-thread_local std::vector<float> g_hist;
-// Per-stage device timing of the ADMM iterations (events on the solve
-// stream around every kernel): index = Stage.
-enum Stage { kStColFwd = 0, kStSolve, kStColInv, kStRow, kStReduce, kStRowFwd,
-             kNumStages };
-thread_local bool g_stage_on = false;
-thread_local double g_stage_ms[kNumStages];
-thread_local int g_stage_n[kNumStages];
+}  // namespace
 
-int fail(int code, const std::string& msg) {
+int set_error(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
-
-#define SCC_HIP(call)                                                  \
-  do {                                                                 \
-    hipError_t e_ = (call);                                            \
-    if (e_ != hipSuccess) {                                            \
-      return fail(e_ == hipErrorOutOfMemory ? SCC_ERR_NOMEM           \
-                                            : SCC_ERR_HIP,            \
-                  std::string(#call) + ": " + hipGetErrorString(e_)); \
-    }                                                                  \
-  } while (0)
 
 // Radix decomposition for the generic Stockham kernels: 4s, then 2s, then
 // odd factors in increasing order.
@@ -86,6 +66,31 @@ bool make_plan(int L, int tw_stride, FftPlan* pl) {
   }
   return true;
 }
+
+namespace {
+
+thread_local scc_info g_info;
+thread_local bool g_hist_on = false;
+thread_local std::vector<float> g_hist;
+// Per-stage device timing of the ADMM iterations (events on the solve
+// stream around every kernel): index = Stage.
+enum Stage { kStColFwd = 0, kStSolve, kStColInv, kStRow, kStReduce, kStRowFwd,
+             kNumStages };
+thread_local bool g_stage_on = false;
+thread_local double g_stage_ms[kNumStages];
+thread_local int g_stage_n[kNumStages];
+
+int fail(int code, const std::string& msg) { return set_error(code, msg); }
+
+#define SCC_HIP(call)                                                  \
+  do {                                                                 \
+    hipError_t e_ = (call);                                            \
+    if (e_ != hipSuccess) {                                            \
+      return fail(e_ == hipErrorOutOfMemory ? SCC_ERR_NOMEM           \
+                                            : SCC_ERR_HIP,            \
+                  std::string(#call) + ": " + hipGetErrorString(e_)); \
+    }                                                                  \
+  } while (0)
 
 struct DevBuf {
   void* p = nullptr;

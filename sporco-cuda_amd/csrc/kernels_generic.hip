@@ -14,180 +14,11 @@
 //                     residual partial sums; Y-U -> row forward FFT
 //   k_reduce          residual norms, stopping test, AutoRho (device side)
 
-#include "scc_internal.h"
-#include "scc_device.h"
+#include "generic_common.h"
 
 namespace scc {
 
-// --------------------------------------------------------------------------
-// Mixed-radix Stockham FFT over nt transforms of length pl.L held in LDS at
-// A + t*ld.  Ping-pongs between A and B and returns the buffer holding the
-// result.  dir = +1 forward (W = exp(-2 pi i / L)), -1 inverse (unnormalised).
-// Every thread of the block must call it (it contains block barriers).
-// --------------------------------------------------------------------------
-__device__ float2* fft_generic(float2* A, float2* B, int nt, int ld,
-                               const FftPlan& pl,
-                               const float2* __restrict__ tw, float dir,
-                               int tid, int nthr) {
-  const int L = pl.L;
-  const int st = pl.tw_stride;
-  int Ns = 1;
-  for (int s = 0; s < pl.nstages; ++s) {
-    const int Rx = pl.radix[s];
-    const int nb = L / Rx;
-    const int tstep = L / (Ns * Rx);
-    const int total = nt * nb;
-    for (int idx = tid; idx < total; idx += nthr) {
-      const int t = idx / nb;
-      const int j = idx - t * nb;
-      const float2* in = A + t * ld;
-      float2* out = B + t * ld;
-      const int k = j % Ns;
-      const int ob = (j - k) * Rx + k;
-      if (Rx == 2) {
-        float2 x0 = in[j];
-        float2 x1 = in[j + nb];
-        if (k) x1 = cmul(x1, twiddle(tw, k * tstep * st, dir));
-        out[ob] = cadd(x0, x1);
-        out[ob + Ns] = csub(x0, x1);
-      } else if (Rx == 4) {
-        float2 x0 = in[j];
-        float2 x1 = in[j + nb];
-        float2 x2 = in[j + 2 * nb];
-        float2 x3 = in[j + 3 * nb];
-        if (k) {
-          x1 = cmul(x1, twiddle(tw, k * tstep * st, dir));
-          x2 = cmul(x2, twiddle(tw, 2 * k * tstep * st, dir));
-          x3 = cmul(x3, twiddle(tw, 3 * k * tstep * st, dir));
-        }
-        float2 s02 = cadd(x0, x2), d02 = csub(x0, x2);
-        float2 s13 = cadd(x1, x3), d13 = csub(x1, x3);
-        float2 jd = mul_mi(d13, dir);  // -i*dir*d13
-        out[ob] = cadd(s02, s13);
-        out[ob + Ns] = cadd(d02, jd);
-        out[ob + 2 * Ns] = csub(s02, s13);
-        out[ob + 3 * Ns] = csub(d02, jd);
-      } else {
-        for (int q = 0; q < Rx; ++q) {
-          const int cq = k * tstep + q * nb;  // < L
-          float2 acc = make_float2(0.f, 0.f);
-          int e = 0;
-          for (int r = 0; r < Rx; ++r) {
-            acc = cadd(acc, cmul(in[j + r * nb], twiddle(tw, e * st, dir)));
-            e += cq;
-            if (e >= L) e -= L;
-          }
-          out[ob + q * Ns] = acc;
-        }
-      }
-    }
-    __syncthreads();
-    float2* tmp = A;
-    A = B;
-    B = tmp;
-    Ns *= Rx;
-  }
-  return A;
-}
-
-// Element i of the (nt rows x H columns) half-spectrum block of rows
-// row0..row0+nt-1, enumerated panel by panel so that consecutive i touch
-// consecutive addresses when the rows are consecutive rows of one map.
-__device__ inline void panel_index(const Geom& g, int nt, int i, int& t,
-                                   int& c) {
-  const int full = (g.P - 1) * nt * g.TC;
-  if (i < full) {
-    const int per = nt * g.TC;
-    const int p = i / per;
-    const int rem = i - p * per;
-    t = rem / g.TC;
-    c = p * g.TC + (rem - t * g.TC);
-  } else {
-    const int rem = i - full;
-    t = rem / g.wlast;
-    c = (g.P - 1) * g.TC + (rem - t * g.wlast);
-  }
-}
-
-__device__ inline long long spec_addr(const Geom& g, long long row, int c) {
-  const long long m = row / g.R;
-  const int r = (int)(row - m * g.R);
-  return m * g.Hc + panel_off(g, r, c);
-}
-
-// Inverse-real-FFT pre-processing of one block of half spectra (in B) into
-// the complex input of the length-Lr inverse transform (in A).
-__device__ void c2r_prep(const Geom& g, const float2* B, float2* A, int nt,
-                         int ld, const float2* __restrict__ twC, int tid,
-                         int nthr) {
-  if (g.even) {
-    const int L = g.Lr;
-    for (int i = tid; i < nt * L; i += nthr) {
-      const int t = i / L;
-      const int k = i - t * L;
-      float2 xk = B[t * ld + k];
-      float2 xn = B[t * ld + L - k];
-      if (k == 0) {  // DC and Nyquist are real (C2R ignores their imag)
-        xk.y = 0.f;
-        xn.y = 0.f;
-      }
-      xn.y = -xn.y;
-      const float2 s = cadd(xk, xn);
-      const float2 wd = cmul(cconj(twC[k]), csub(xk, xn));
-      A[t * ld + k] = make_float2(s.x - wd.y, s.y + wd.x);  // s + i*wd
-    }
-  } else {
-    const int C = g.C;
-    for (int i = tid; i < nt * C; i += nthr) {
-      const int t = i / C;
-      const int k = i - t * C;
-      float2 v = (k < g.H) ? B[t * ld + k] : cconj(B[t * ld + C - k]);
-      if (k == 0) v.y = 0.f;
-      A[t * ld + k] = v;
-    }
-  }
-}
-
-// Forward-real-FFT post-processing of the length-Lr transform (in Z) and
-// store of the half spectra to F in panel layout.
-__device__ void r2c_store(const Geom& g, const float2* Z, int nt, int ld,
-                          long long row0, float2* __restrict__ F,
-                          const float2* __restrict__ twC, int tid, int nthr) {
-  const int L = g.Lr;
-  for (int i = tid; i < nt * g.H; i += nthr) {
-    int t, k;
-    panel_index(g, nt, i, t, k);
-    float2 X;
-    if (g.even) {
-      const float2 zk = Z[t * ld + (k == L ? 0 : k)];
-      const float2 zn = cconj(Z[t * ld + (k == 0 ? 0 : L - k)]);
-      const float2 s = cadd(zk, zn);
-      const float2 wd = cmul(twC[k], csub(zk, zn));
-      X = make_float2(0.5f * (s.x + wd.y), 0.5f * (s.y - wd.x));
-    } else {
-      X = Z[t * ld + k];
-    }
-    F[spec_addr(g, row0 + t, k)] = X;
-  }
-}
-
-__device__ void spec_load(const Geom& g, const float2* __restrict__ F,
-                          float2* B, int nt, int ld, long long row0, int tid,
-                          int nthr) {
-  for (int i = tid; i < nt * g.H; i += nthr) {
-    int t, c;
-    panel_index(g, nt, i, t, c);
-    B[t * ld + c] = F[spec_addr(g, row0 + t, c)];
-  }
-}
-
-static inline int row_nr(const Geom& g) {
-  const int ld = g.Lr + 1;
-  int nr = 61440 / (2 * ld * 8);
-  if (nr > 16) nr = 16;
-  if (nr < 1) nr = 1;
-  return nr;
-}
+static inline int row_nr(const Geom& g) { return row_nr_budget(g, 61440); }
 
 // --------------------------------------------------------------------------
 // Filter padding (cuda_Pad_Dict, cbpdn_kernels.cu:602-615): origin at [0,0].
@@ -214,36 +45,9 @@ __global__ __launch_bounds__(256) void k_row_fwd_generic(
     const Ctrl* ctrl, int nr) {
   if (mode == kFwdYU && ctrl->done) return;
   extern __shared__ float2 lds[];
-  const int ld = g.Lr + 1;
-  float2* A = lds;
-  float2* B = lds + nr * ld;
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  const long long row0 = (long long)blockIdx.x * nr;
-  const int nt = (int)min((long long)nr, nrows - row0);
   const float rsf = (mode == kFwdYU) ? ctrl->rsf : 1.f;
-  for (int i = tid; i < nt * g.C; i += nthr) {
-    const int t = i / g.C;
-    const int n = i - t * g.C;
-    const long long gi = (row0 + t) * g.C + n;
-    float a;
-    if (mode == kFwdReal) {
-      a = src[gi];
-    } else {
-      float u = U[gi];
-      if (rsf != 1.f) {
-        u = u / rsf;
-        U[gi] = u;
-      }
-      a = Y[gi] - u;
-    }
-    if (g.even)
-      reinterpret_cast<float*>(A + t * ld)[n] = a;
-    else
-      A[t * ld + n] = make_float2(a, 0.f);
-  }
-  __syncthreads();
-  const float2* Z = fft_generic(A, B, nt, ld, rp, twC, 1.f, tid, nthr);
-  r2c_store(g, Z, nt, ld, row0, F, twC, tid, nthr);
+  row_fwd_body(g, nrows, rp, mode, src, Y, U, F, twC, rsf, nr, blockIdx.x,
+               lds);
 }
 
 // --------------------------------------------------------------------------
@@ -254,27 +58,7 @@ __global__ __launch_bounds__(256) void k_col_generic(
     float dir, const Ctrl* ctrl) {
   if (ctrl != nullptr && ctrl->done) return;
   extern __shared__ float2 lds[];
-  const int p = blockIdx.x;
-  const long long m = blockIdx.y;
-  const int w = (p == g.P - 1) ? g.wlast : g.TC;
-  const int ld = g.R + 1;
-  float2* A = lds;
-  float2* B = lds + w * ld;
-  float2* base = F + m * g.Hc + (long long)p * g.TC * g.R;
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  const int n = g.R * w;
-  for (int i = tid; i < n; i += nthr) {
-    const int r = i / w;
-    const int c = i - r * w;
-    A[c * ld + r] = base[i];
-  }
-  __syncthreads();
-  const float2* Z = fft_generic(A, B, w, ld, cp, twR, dir, tid, nthr);
-  for (int i = tid; i < n; i += nthr) {
-    const int r = i / w;
-    const int c = i - r * w;
-    base[i] = Z[c * ld + r];
-  }
+  col_body(g, cp, F, twR, dir, blockIdx.x, blockIdx.y, lds);
 }
 
 // --------------------------------------------------------------------------
@@ -334,48 +118,7 @@ __global__ __launch_bounds__(256) void k_row_generic(Geom g, FftPlan rp,
   if (b.ctrl->done) return;
   extern __shared__ float2 lds[];
   __shared__ float red[5][4];
-  const int ld = g.Lr + 1;
-  float2* A = lds;
-  float2* B = lds + nr * ld;
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  const long long nrows = (long long)g.M * g.R;
-  const long long row0 = (long long)blockIdx.x * nr;
-  const int nt = (int)min((long long)nr, nrows - row0);
-  const float rho = b.ctrl->rho;
-
-  spec_load(g, b.F, B, nt, ld, row0, tid, nthr);
-  __syncthreads();
-  c2r_prep(g, B, A, nt, ld, b.twC, tid, nthr);
-  __syncthreads();
-  float2* P = fft_generic(A, B, nt, ld, rp, b.twC, -1.f, tid, nthr);
-  float2* Q = (P == A) ? B : A;
-
-  UpdateSums acc;
-  const float invN = 1.0f / (float)g.N;
-  const float lr = p.lambda / rho;
-  for (int i = tid; i < nt * g.C; i += nthr) {
-    const int t = i / g.C;
-    const int n = i - t * g.C;
-    const long long row = row0 + t;
-    const int m = (int)(row / g.R);
-    const int r = (int)(row - (long long)m * g.R);
-    const float xv = g.even ? reinterpret_cast<const float*>(P + t * ld)[n]
-                            : P[t * ld + n].x;
-    const long long gi = row * g.C + n;
-    float a = update_point(g, p, b, xv * invN, lr, m, r, n, gi, acc);
-    if (mode == kRowFused) {
-      if (g.even)
-        reinterpret_cast<float*>(P + t * ld)[n] = a;
-      else
-        P[t * ld + n] = make_float2(a, 0.f);
-    }
-  }
-  block_sums_store(acc, red, b.partials, gridDim.x, blockIdx.x, tid);
-  if (mode == kRowFused) {
-    __syncthreads();
-    const float2* Z = fft_generic(P, Q, nt, ld, rp, b.twC, 1.f, tid, nthr);
-    r2c_store(g, Z, nt, ld, row0, b.F, b.twC, tid, nthr);
-  }
+  row_body(g, rp, b, p, mode, nr, blockIdx.x, gridDim.x, lds, red);
 }
 
 // --------------------------------------------------------------------------

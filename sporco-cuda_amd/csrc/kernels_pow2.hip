@@ -14,6 +14,7 @@
 //                 float4 -> Y - U -> forward real FFT -> spectrum.
 
 #include "fft_reg.h"
+#include "pow2_common.h"
 #include "scc_device.h"
 #include "scc_pow2.h"
 
@@ -107,10 +108,12 @@ __global__ __launch_bounds__(256) void k_solve_pow2(Geom g, Buffers b,
   const float rho = b.ctrl->rho;
   const float irho = 1.f / rho;
   const int M = g.M;
+  // Images of a batch (same size, shared dictionary): blockIdx.y.
+  float2* const Fi = b.F + (long long)blockIdx.y * M * g.Hc;
   float2 sh = make_float2(0.f, 0.f);
   float gh = 0.f;
   if (valid) {
-    sh = b.Sf[f];
+    sh = b.Sf[(long long)blockIdx.y * g.Hc + f];
     if (GRAD) gh = p.mu * b.ghg[f];
   }
   // b_m and D_m stay in registers between the two passes; 1/delta_m too
@@ -125,7 +128,7 @@ __global__ __launch_bounds__(256) void k_solve_pow2(Geom g, Buffers b,
     if (valid && m < M) {
       const long long o = (long long)m * g.Hc + f;
       const float2 d = b.Df[o];
-      const float2 a = b.F[o];
+      const float2 a = Fi[o];
       const float2 x = cadd(cmul(cconj(d), sh), cscale(a, rho));
       float inv = irho;
       if (GRAD) {
@@ -150,7 +153,7 @@ __global__ __launch_bounds__(256) void k_solve_pow2(Geom g, Buffers b,
     const int m = gi * MPT + i;
     if (valid && m < M) {
       const float inv = GRAD ? id[i] : irho;
-      b.F[(long long)m * g.Hc + f] =
+      Fi[(long long)m * g.Hc + f] =
           cscale(csub(bb[i], cmul(cconj(dd[i]), cb)), inv);
     }
   }
@@ -167,59 +170,6 @@ __global__ __launch_bounds__(256) void k_solve_pow2(Geom g, Buffers b,
 // order of the real row as float2 pairs, so the update and the forward
 // transform of Y - U need no LDS pass at all.
 // --------------------------------------------------------------------------
-template <int LR>
-struct RowCfg {
-  static constexpr int E = LR <= 512 ? 8 : 16;
-  static constexpr int T = LR / E;       // lanes per row
-  static constexpr int RPW = 64 / T;     // rows per wavefront
-  static constexpr int RPB = 4 * RPW;    // rows per 256-thread block
-  // Y and U are loaded with the spectrum (latency hidden behind the inverse
-  // transform) where registers allow it.
-  static constexpr bool PREFETCH = E <= 8;
-  // Occupancy target for the register allocator (4 waves per SIMD fits the
-  // E = 8 plain update without spills).
-  static constexpr int WAVES = E <= 8 ? 4 : 2;
-};
-
-__device__ __forceinline__ float2 shfl2(float2 v, int src) {
-  return make_float2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
-}
-
-// Offset of half-spectrum column k of row r in a map (panel layout with
-// full panels of TC columns and a last panel holding column LR).
-template <int TC>
-__device__ __forceinline__ int spec_off(int k, int r, int R) {
-  return (k / TC) * (TC * R) + r * TC + (k % TC);
-}
-
-// X[k] from the forward transform Z of the packed real row:
-// X[k] = (Z[k] + conj Z[L-k]) / 2 - i W_C^k (Z[k] - conj Z[L-k]) / 2.
-__device__ __forceinline__ float2 r2c_post(float2 zk, float2 znc, float2 w) {
-  const float2 s = cadd(zk, znc);
-  const float2 wd = cmul(w, csub(zk, znc));
-  return make_float2(0.5f * (s.x + wd.y), 0.5f * (s.y - wd.x));
-}
-
-template <int LR, int TC>
-__device__ __forceinline__ void store_spectrum(const float2 (&z)[RowCfg<LR>::E],
-                                               int t, int partner, int r,
-                                               int R, float2* Fm,
-                                               const float2* twC) {
-  using RC = RowCfg<LR>;
-  constexpr int E = RC::E, T = RC::T;
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int k = t + T * e;
-    const float2 sh = shfl2(z[E - 1 - e], partner);
-    const float2 zn = (t == 0) ? (e == 0 ? z[0] : z[(E - e) % E]) : sh;
-    Fm[spec_off<TC>(k, r, R)] = r2c_post(z[e], cconj(zn), twC[k]);
-  }
-  if (t == 0)  // column L: Re Z[0] - Im Z[0]
-    Fm[(LR / TC) * TC * R + r] = make_float2(z[0].x - z[0].y, 0.f);
-}
-
-// GENERAL selects the update with an AMS mask, a full weight array,
-// NonNegCoef or NoBndryCross; without it the threshold is one value per row.
 template <int LR, int TC, bool GENERAL>
 __global__ __launch_bounds__(256) __attribute__((
     amdgpu_waves_per_eu(GENERAL ? 1 : RowCfg<LR>::WAVES))) void
@@ -340,6 +290,9 @@ k_row_pow2(Geom g, Buffers b, Params p, int mode) {
       const float yn0 = copysignf(fmaxf(fabsf(v0) - th, 0.f), v0);
       const float yn1 = copysignf(fmaxf(fabsf(v1) - th, 0.f), v1);
       const float un0 = v0 - yn0, un1 = v1 - yn1;
+      if (b.vout != nullptr)
+        reinterpret_cast<float2*>(b.vout + (long long)row0 * C)[lo + T * e] =
+            make_float2(v0, v1);
       acc.x = fmaf(x0, x0, fmaf(x1, x1, acc.x));
       acc.y = fmaf(yn0, yn0, fmaf(yn1, yn1, acc.y));
       acc.u = fmaf(un0, un0, fmaf(un1, un1, acc.u));
@@ -520,10 +473,10 @@ hipError_t pow2_col(const Geom& g, const Pow2Config& cfg, int nmaps,
 }
 
 hipError_t pow2_solve(const Geom& g, const Pow2Config& cfg, const Buffers& b,
-                      const Params& p, hipStream_t st) {
+                      const Params& p, hipStream_t st, int nimg) {
   const int G = cfg.variant;
   const long long fpb = 4LL * (64 / G);  // frequencies per 256-thread block
-  const unsigned nb = (unsigned)((g.Hc + fpb - 1) / fpb);
+  const dim3 nb((unsigned)((g.Hc + fpb - 1) / fpb), (unsigned)nimg);
 #define X(gg)                                                              \
   if (G == gg) {                                                           \
     if (p.grad)                                                            \

@@ -1,0 +1,117 @@
+// Helpers shared by the power-of-two row kernels of the solver
+// (kernels_pow2.hip) and of its adjoint (unrolled.hip).
+#pragma once
+
+#include "fft_reg.h"
+#include "scc_device.h"
+
+namespace scc {
+
+// Every row belongs to one wavefront (RPW rows per wavefront, T lanes per
+// row, E points per lane).  The half spectrum is read and written in slot
+// order (lane t holds columns t + T e); the real-FFT pre/post-processing
+// pairs column k with column L - k, which lives in lane (T - t) mod T, slot
+// E - 1 - e (lane 0: its own slot E - e, or column L), fetched with a lane
+// shuffle.
+template <int LR>
+struct RowCfg {
+  static constexpr int E = LR <= 512 ? 8 : 16;
+  static constexpr int T = LR / E;       // lanes per row
+  static constexpr int RPW = 64 / T;     // rows per wavefront
+  static constexpr int RPB = 4 * RPW;    // rows per 256-thread block
+  // Y and U are loaded with the spectrum (latency hidden behind the inverse
+  // transform) where registers allow it.
+  static constexpr bool PREFETCH = E <= 8;
+  // Occupancy target for the register allocator (4 waves per SIMD fits the
+  // E = 8 plain update without spills).
+  static constexpr int WAVES = E <= 8 ? 4 : 2;
+};
+
+__device__ __forceinline__ float2 shfl2(float2 v, int src) {
+  return make_float2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
+}
+
+// Offset of half-spectrum column k of row r in a map (panel layout with
+// full panels of TC columns and a last panel holding column LR).
+template <int TC>
+__device__ __forceinline__ int spec_off(int k, int r, int R) {
+  return (k / TC) * (TC * R) + r * TC + (k % TC);
+}
+
+// X[k] from the forward transform Z of the packed real row:
+// X[k] = (Z[k] + conj Z[L-k]) / 2 - i W_C^k (Z[k] - conj Z[L-k]) / 2.
+__device__ __forceinline__ float2 r2c_post(float2 zk, float2 znc, float2 w) {
+  const float2 s = cadd(zk, znc);
+  const float2 wd = cmul(w, csub(zk, znc));
+  return make_float2(0.5f * (s.x + wd.y), 0.5f * (s.y - wd.x));
+}
+
+template <int LR, int TC>
+__device__ __forceinline__ void store_spectrum(const float2 (&z)[RowCfg<LR>::E],
+                                               int t, int partner, int r,
+                                               int R, float2* Fm,
+                                               const float2* twC) {
+  using RC = RowCfg<LR>;
+  constexpr int E = RC::E, T = RC::T;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = t + T * e;
+    const float2 sh = shfl2(z[E - 1 - e], partner);
+    const float2 zn = (t == 0) ? (e == 0 ? z[0] : z[(E - e) % E]) : sh;
+    Fm[spec_off<TC>(k, r, R)] = r2c_post(z[e], cconj(zn), twC[k]);
+  }
+  if (t == 0)  // column L: Re Z[0] - Im Z[0]
+    Fm[(LR / TC) * TC * R + r] = make_float2(z[0].x - z[0].y, 0.f);
+}
+
+// GENERAL selects the update with an AMS mask, a full weight array,
+// NonNegCoef or NoBndryCross; without it the threshold is one value per row.
+
+// Load the half spectrum of row r (slot order, column L on lane 0) and form
+// the input of the length-LR inverse transform of the packed real row:
+// Z'[k] = (X[k] + conj X[L-k]) + i W_C^-k (X[k] - conj X[L-k]).
+template <int LR, int TC>
+__device__ __forceinline__ void load_spectrum_c2r(float2 (&z)[RowCfg<LR>::E],
+                                                  int t, int partner, int r,
+                                                  int R, const float2* Fm,
+                                                  const float2* twC) {
+  using RC = RowCfg<LR>;
+  constexpr int E = RC::E, T = RC::T;
+  float2 v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = Fm[spec_off<TC>(t + T * e, r, R)];
+  float2 xL = make_float2(0.f, 0.f);
+  if (t == 0) xL = Fm[(LR / TC) * TC * R + r];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = t + T * e;
+    const float2 sh = shfl2(v[E - 1 - e], partner);
+    float2 xn = (t == 0) ? (e == 0 ? xL : v[(E - e) % E]) : sh;
+    float2 xk = v[e];
+    if (t == 0 && e == 0) {
+      xk.y = 0.f;
+      xn.y = 0.f;
+    }
+    xn.y = -xn.y;
+    const float2 s = cadd(xk, xn);
+    const float2 wd = cmul(cconj(twC[k]), csub(xk, xn));
+    z[e] = make_float2(s.x - wd.y, s.y + wd.x);
+  }
+}
+
+// Column index of flat half-spectrum offset f (< Hc) in the panel layout.
+__device__ __forceinline__ int panel_col(const Geom& g, long long f) {
+  const int pr = g.TC * g.R;
+  const int p = (int)(f / pr);
+  const int o = (int)(f - (long long)p * pr);
+  const int w = (p == g.P - 1) ? g.wlast : g.TC;
+  return p * g.TC + o % w;
+}
+
+// Multiplicity of a half-spectrum column in the full spectrum: 1 for DC
+// (and Nyquist when C is even), 2 for the others.
+__device__ __forceinline__ float col_mult(const Geom& g, int c) {
+  return (c == 0 || (g.even && c == g.C / 2)) ? 1.f : 2.f;
+}
+
+}  // namespace scc

@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 namespace scc {
 
 constexpr int kMaxStages = 24;
@@ -102,6 +104,7 @@ struct Buffers {
   float* hist;      // 3 x max_iter history (r, s, rho), optional
   float2* twR;      // W_R^t, t < R
   float2* twC;      // W_C^t, t < C
+  float* vout;      // optional copy of V = Xr + U (unrolled solve tape)
 };
 
 // Kernel launchers (kernels.hip).
@@ -122,5 +125,10 @@ hipError_t launch_row(const Geom& g, const FftPlan& rp, const Buffers& b,
 hipError_t launch_reduce(const Buffers& b, const Params& p, int nblocks,
                          int iter, int max_iter, hipStream_t st);
 int row_blocks(const Geom& g);
+
+// Host helpers of the driver (api.hip): the thread's last-error message
+// returned by scc_last_error, and the mixed-radix plan of one FFT length.
+int set_error(int code, const std::string& msg);
+bool make_plan(int L, int tw_stride, FftPlan* pl);
 
 }  // namespace scc

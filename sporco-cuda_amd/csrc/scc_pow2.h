@@ -22,8 +22,10 @@ hipError_t pow2_row_fwd(const Geom& g, const Pow2Config& cfg, int nmaps,
 hipError_t pow2_col(const Geom& g, const Pow2Config& cfg, int nmaps,
                     float2* F, const float2* twR, int inverse,
                     const Ctrl* ctrl, hipStream_t st);
+// nimg same-size images (maps of image i at F + i M Hc, spectrum at
+// Sf + i Hc) share the dictionary spectra.
 hipError_t pow2_solve(const Geom& g, const Pow2Config& cfg, const Buffers& b,
-                      const Params& p, hipStream_t st);
+                      const Params& p, hipStream_t st, int nimg = 1);
 hipError_t pow2_row(const Geom& g, const Pow2Config& cfg, const Buffers& b,
                     const Params& p, int mode, int* nblocks, hipStream_t st);
 

@@ -43,6 +43,26 @@ class AlgOpt(ctypes.Structure):
     ]
 
 
+class BatchImage(ctypes.Structure):
+    """Mirror of scc_batch_image."""
+    _fields_ = [('R', _i), ('C', _i), ('S', ctypes.c_void_p),
+                ('Y', ctypes.c_void_p), ('lmbda', _f), ('rho', _f)]
+
+
+class BatchResult(ctypes.Structure):
+    """Mirror of scc_batch_result."""
+    _fields_ = [('iterations', _i), ('rho', _f), ('r', _f), ('s', _f),
+                ('epri', _f), ('edua', _f)]
+
+
+class BatchConfig(ctypes.Structure):
+    """Mirror of scc_batch_config."""
+    _fields_ = [('col_lds', _i), ('row_lds', _i), ('solve_lanes', _i),
+                ('solve_maps', _i), ('persistent', _i), ('graph', _i),
+                ('tune_ms', ctypes.c_double), ('setup_ms', ctypes.c_double),
+                ('solve_ms', ctypes.c_double)]
+
+
 class Info(ctypes.Structure):
     _fields_ = [('iterations', _i), ('rho', _f), ('r', _f), ('s', _f),
                 ('epri', _f), ('edua', _f), ('setup_ms', ctypes.c_double),
@@ -78,6 +98,19 @@ def lib():
         'scc_get_device_name': (ctypes.c_char_p, [_i]),
         'scc_release_cache': (None, []),
         'scc_set_stage_timing': (None, [_i]),
+        'scc_unrolled_ckpt_floats': (ctypes.c_size_t, [_i, _i, _i, _i, _i,
+                                                       _i]),
+        'scc_unrolled_forward': (_i, [vp, _i, _i, vp, _i, _i, _i, _f, _f, _f,
+                                      _i, _i, vp, vp, vp]),
+        'scc_unrolled_backward': (_i, [vp, _i, _i, vp, _i, _i, _i, _f, _f,
+                                       _f, _i, _i, vp, vp, vp, vp, vp, vp]),
+        'scc_batched_cbpdn': (_i, [vp, _i, _i, _i, _i, vp, vp, vp, vp]),
+        'scc_batched_set_autotune': (_i, [_i]),
+        'scc_batched_set_graph': (_i, [_i]),
+        'scc_batched_last_config': (_i, [vp]),
+        'scc_unrolled_tape': (_i, [vp, _i, _i, vp, _i, _i, _i, _f, _f, _f,
+                                   _i, vp, vp]),
+        'scc_unrolled_last_error': (ctypes.c_char_p, []),
         'scc_last_stage_times': (_i, [vp, vp, _i]),
     }
     for name, (res, args) in sig.items():

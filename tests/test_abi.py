@@ -163,3 +163,83 @@ def test_device_queries_without_device():
         assert cu.device_name() is None
         with pytest.raises(RuntimeError):
             cu.memory_info()
+
+
+@pytest.mark.parametrize('name', ['scc_batch_image', 'scc_batch_result',
+                                  'scc_batch_config'])
+def test_batch_struct_layouts(name):
+    """ctypes mirrors of the batched structs against the C compiler."""
+    import shutil
+    import subprocess
+    import tempfile
+    from sporco_cuda import _abi
+    cls = {'scc_batch_image': _abi.BatchImage,
+           'scc_batch_result': _abi.BatchResult,
+           'scc_batch_config': _abi.BatchConfig}[name]
+    cnames = [f for f, _ in cls._fields_]
+    if shutil.which('gcc') is None:
+        pytest.skip('no C compiler')
+    # ctypes names follow the header except lambda (a Python keyword).
+    cfields = ['lambda' if f == 'lmbda' else f for f in cnames]
+    src = '#include <stdio.h>\n#include <stddef.h>\n#include "sporco_amd.h"\n'
+    src += 'int main(void){printf("%%zu\\n", sizeof(%s));' % name
+    for f in cfields:
+        src += 'printf("%%zu\\n", offsetof(%s, %s));' % (name, f)
+    src += 'return 0;}\n'
+    with tempfile.TemporaryDirectory() as d:
+        cf = os.path.join(d, 'l.c')
+        open(cf, 'w').write(src)
+        exe = os.path.join(d, 'l')
+        subprocess.check_call(['gcc', '-I', os.path.dirname(HEADER), cf,
+                               '-o', exe])
+        out = [int(v) for v in subprocess.check_output([exe]).split()]
+    assert out[0] == ctypes.sizeof(cls)
+    assert out[1:] == [getattr(cls, f).offset for f in cnames]
+
+
+def _batched_call(nimg=2, M=3, K=3, sizes=((16, 12), (9, 20)), lmbda=0.1,
+                  img_rho=1.0, **opt_fields):
+    from sporco_cuda import _abi
+    L = _abi.lib()
+    D = np.ones((M, K, K), np.float32)
+    S = [np.ones(s, np.float32) for s in sizes]
+    Y = [np.zeros((M,) + s, np.float32) for s in sizes]
+    w = np.ones(1, np.float32)
+    o = _abi.make_opt(8, 8, M, K, K, 1.0, 5, l1_ptr=w.ctypes.data)
+    for k, v in opt_fields.items():
+        setattr(o, k, v)
+    arr = (_abi.BatchImage * max(nimg, 1))()
+    for i in range(min(nimg, len(sizes))):
+        arr[i].R, arr[i].C = sizes[i]
+        arr[i].S, arr[i].Y = S[i].ctypes.data, Y[i].ctypes.data
+        arr[i].lmbda = lmbda
+        arr[i].rho = img_rho
+    rc = L.scc_batched_cbpdn(D.ctypes.data, M, K, K, nimg, arr,
+                             ctypes.byref(o), None, None)
+    return rc, _abi.last_error()
+
+
+@pytest.mark.parametrize('kw,code', [
+    (dict(nimg=0), 'INVALID'),
+    (dict(sizes=((16, 12), (2, 20))), 'INVALID'),        # filter > image
+    (dict(lmbda=float('nan')), 'INVALID'),
+    (dict(img_rho=-1.0, rho=0.0), 'INVALID'),
+    (dict(img_rho=float('inf')), 'INVALID'),
+    (dict(MaxMainIter=-3), 'INVALID'),
+    (dict(nWeight=5), 'UNSUPPORTED'),                    # masks
+    (dict(L1_WEIGHT_M_SIZE=3, L1_WEIGHT_ROW_SIZE=4), 'UNSUPPORTED'),
+    (dict(sizes=((16, 12), (5000, 20))), 'UNSUPPORTED'),
+])
+def test_batched_rejects_before_device(kw, code):
+    from sporco_cuda import _abi
+    rc, msg = _batched_call(**kw)
+    assert rc == getattr(_abi, 'SCC_ERR_' + code), (rc, msg)
+    assert msg
+
+
+def test_batched_no_device_is_a_status():
+    from sporco_cuda import _abi
+    if _abi.lib().scc_get_device_count() > 0:
+        pytest.skip('a device is present')
+    rc, _ = _batched_call()
+    assert rc == _abi.SCC_ERR_NODEVICE

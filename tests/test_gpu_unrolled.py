@@ -1,0 +1,126 @@
+"""GPU unrolled CBPDN (forward kernels + hand-written adjoint kernels)
+against the FP64 references.
+
+The unrolled iteration is piecewise linear: its gradient jumps where some
+|V| crosses the soft threshold t = lambda / rho.  On 2 x 4 x 256 x 256
+coefficients a few values of V lie within FP32 rounding (~1e-6) of t, and
+there the FP32 GPU forward and the FP64 oracle can take different branches;
+one flipped support moves the gradient by about 1 / sqrt(#coefficients)
+(~1e-3 relative), so gradients are compared on the branch the GPU took:
+  * forward Y against the FP64 oracle (relative L2 <= max(1e-4, 2 e32));
+  * the GPU's V of every iteration (scc_unrolled_tape) against FP64, and
+    every support that differs lies within 1e-5 of t (a rounding flip);
+  * gradients w.r.t. D, S, lambda against the FP64 hand adjoint on the
+    GPU's supports (relative error <= 1e-4);
+  * the same gradients against FP64 central differences (eps = 1e-3) of
+    the FP64 loss on the GPU's supports (smooth there), along random
+    unit-norm directions (relative error <= 1e-3);
+  * against the FP64 adjoint on the oracle's own supports the error is
+    bounded by the flips (<= 2e-2, printed);
+  * the checkpoint interval does not change the gradient beyond rounding.
+"""
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip('torch')
+
+from oracle import unrolled as ur  # noqa: E402
+
+R, C, M, K, B = 256, 256, 4, 5, 2
+LAM, RHO, ALPHA, NIT = 0.05, 2.0, 1.8, 12
+
+
+def _inputs(seed=7):
+    rng = np.random.default_rng(seed)
+    D = rng.standard_normal((M, K, K))
+    D /= np.sqrt(np.sum(D ** 2, axis=(1, 2), keepdims=True))
+    S = rng.standard_normal((B, R, C))
+    G = rng.standard_normal((B, M, R, C))
+    return D.astype(np.float32), S.astype(np.float32), G.astype(np.float32)
+
+
+def _gpu_grads(D, S, G, ck):
+    from sporco_cuda.unrolled import cbpdn_unrolled
+    dev = torch.device('cuda:0')
+    Dt = torch.tensor(D, device=dev, requires_grad=True)
+    St = torch.tensor(S, device=dev, requires_grad=True)
+    lt = torch.tensor(LAM, device=dev, requires_grad=True)
+    Y = cbpdn_unrolled(Dt, St, lt, rho=RHO, n_iter=NIT, relax=ALPHA,
+                       ckpt_every=ck)
+    (Y * torch.tensor(G, device=dev)).sum().backward()
+    return (Y.detach().cpu().numpy(), Dt.grad.cpu().numpy(),
+            St.grad.cpu().numpy(), lt.grad.item())
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b)
+
+
+@pytest.mark.gpu
+def test_unrolled_forward_and_gradients(gpu):
+    from sporco_cuda.unrolled import unrolled_tape
+    D, S, G = _inputs()
+    print('unrolled', (B, M, R, C), NIT, flush=True)
+    Y, gD, gS, gl = _gpu_grads(D, S, G, 5)
+    D64, S64, G64 = D.astype(np.float64), S.astype(np.float64), \
+        G.astype(np.float64)
+    thr = LAM / RHO
+    Yo, st = ur.forward_numpy(D64, S64, LAM, RHO, NIT, ALPHA)
+    Y32, _ = ur.forward_numpy(D.astype(np.float32), S.astype(np.float32),
+                              np.float32(LAM), np.float32(RHO), NIT,
+                              np.float32(ALPHA))
+    assert rel(Y, Yo) <= max(1e-4, 2 * rel(Y32, Yo))
+    # The GPU's branch: V of every iteration, supports, flips at rounding.
+    dev = torch.device('cuda:0')
+    V = unrolled_tape(torch.tensor(D, device=dev), torch.tensor(S, device=dev),
+                      LAM, RHO, NIT, ALPHA).cpu().numpy()
+    sup = [np.abs(V[i]) > np.float32(thr) for i in range(NIT)]
+    Yg, stg = ur.forward_numpy(D64, S64, LAM, RHO, NIT, ALPHA, supports=sup)
+    nflip = 0
+    for i in range(NIT):
+        Vo = stg['tape'][i][1]
+        assert np.max(np.abs(V[i] - Vo)) <= 1e-5 * np.max(np.abs(Vo))
+        Vn = st['tape'][i][1]
+        flip = sup[i] != (np.abs(Vn) > thr)
+        nflip += int(flip.sum())
+        assert np.all(np.abs(np.abs(Vn[flip]) - thr) <= 1e-5)
+    dref, sref, lref = ur.adjoint_numpy(D64, S64, LAM, RHO, NIT, G64, ALPHA,
+                                        supports=sup)
+    e = (rel(gD, dref), rel(gS, sref), abs(gl - lref) / abs(lref))
+    print('flipped supports %d; rel err on the GPU branch D %.2e S %.2e '
+          'lambda %.2e' % ((nflip,) + e), flush=True)
+    assert max(e) <= 1e-4, e
+    d0, s0, l0 = ur.adjoint_numpy(D64, S64, LAM, RHO, NIT, G64, ALPHA)
+    e0 = (rel(gD, d0), rel(gS, s0), abs(gl - l0) / abs(l0))
+    print('rel err on the oracle branch D %.2e S %.2e lambda %.2e' % e0,
+          flush=True)
+    assert max(e0) <= 2e-2, e0
+    # FP64 central differences (eps = 1e-3) of the loss on the GPU branch.
+
+    def f(Dx, Sx, lx):
+        return ur.loss_and_support(Dx, Sx, lx, RHO, NIT, G64, ALPHA,
+                                   supports=sup)[0]
+    eps = 1e-3
+    rng = np.random.default_rng(11)
+    for _ in range(2):
+        vd = rng.standard_normal(D.shape)
+        vs = rng.standard_normal(S.shape)
+        nv = np.sqrt(np.sum(vd ** 2) + np.sum(vs ** 2))  # unit direction
+        vd, vs = vd / nv, vs / nv
+        fd = (f(D64 + eps * vd, S64 + eps * vs, LAM)
+              - f(D64 - eps * vd, S64 - eps * vs, LAM)) / (2 * eps)
+        an = float(np.sum(gD * vd) + np.sum(gS * vs))
+        assert abs(fd - an) <= 1e-3 * abs(fd), (fd, an)
+    fd = (f(D64, S64, LAM + eps) - f(D64, S64, LAM - eps)) / (2 * eps)
+    assert abs(fd - gl) <= 1e-3 * abs(fd), (fd, gl)
+
+
+@pytest.mark.gpu
+def test_checkpoint_interval_invariance(gpu):
+    D, S, G = _inputs(3)
+    a = _gpu_grads(D, S, G, 1)
+    b = _gpu_grads(D, S, G, NIT)
+    for x, y in zip(a[:3], b[:3]):
+        assert rel(x, y) <= 1e-5
+    assert abs(a[3] - b[3]) <= 1e-5 * abs(b[3])
