@@ -125,6 +125,144 @@ def cpu_baseline(wl, D, S, budget_s):
     }
 
 
+def batched_leg(seed=3, n_images=64, iters=150):
+    """Uneven batch: 64 images with sides drawn from the multiples of 8 in
+    [128, 768], 64 unit-norm 8x8 filters, per-image lambda log-uniform in
+    [0.005, 0.1], rho = 50, 150 fixed iterations.  Times the batched solver
+    (second call: autotuning and graph capture happen on the first) against
+    a loop of single-image device solves of the same images, and compares
+    their final residuals."""
+    import numpy as np
+    import torch
+    import sporco_cuda.cbpdn as cb
+    from sporco_cuda import _abi, batched
+    rng = np.random.default_rng(seed)
+    sizes = [(8 * int(rng.integers(16, 97)), 8 * int(rng.integers(16, 97)))
+             for _ in range(n_images)]
+    lm = np.exp(rng.uniform(np.log(0.005), np.log(0.1), n_images))
+    M, K = 64, 8
+    dev = torch.device('cuda:0')
+    D = torch.from_numpy(synth_dict(K, M, 2000 + seed)).to(dev)
+    imgs = [torch.from_numpy(synth_image(R, C, 1000 + seed + 7 * i)).to(dev)
+            for i, (R, C) in enumerate(sizes)]
+    opt = cb.default_options({'MaxMainIter': iters, 'rho': 50.0,
+                              'RelStopTol': 0.0,
+                              'AutoRho': {'Enabled': False}})
+    print('bench: batched leg, %d images, %d iterations' % (n_images, iters),
+          file=sys.stderr, flush=True)
+    res = []
+    batched.cbpdn_batched_device(D, imgs, lm, opt, results=res)
+    cfg0 = batched.last_config()
+    torch.cuda.synchronize()
+    res = []
+    t0 = time.perf_counter()
+    Yb = batched.cbpdn_batched_device(D, imgs, lm, opt, results=res)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    cfg = batched.last_config()
+    L = _abi.lib()
+    w = torch.ones(1, dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    worst = 0.0
+    ydiff = 0.0
+    t0 = time.perf_counter()
+    for i, (R, C) in enumerate(sizes):
+        Y = torch.empty((M, R, C), dtype=torch.float32, device=dev)
+        o = _abi.make_opt(R, C, M, K, K, 50.0, iters, relax=1.8,
+                          l1_ptr=w.data_ptr())
+        rc = L.scc_cbpdn_device(D.data_ptr(), imgs[i].data_ptr(),
+                                float(lm[i]), 0.0, 0, ctypes.byref(o),
+                                Y.data_ptr(), stream)
+        if rc != 0:
+            raise RuntimeError(_abi.last_error())
+        inf = _abi.last_info()
+        worst = max(worst, abs(inf['r'] - res[i]['r']) / max(inf['r'], 1e-30),
+                    abs(inf['s'] - res[i]['s']) / max(inf['s'], 1e-30))
+        ydiff = max(ydiff, float(torch.linalg.norm(Y - Yb[i]) /
+                                 torch.linalg.norm(Y)))
+    torch.cuda.synchronize()
+    tl = time.perf_counter() - t0
+    N = sum(R * C for R, C in sizes)
+    return {
+        'workload': '%d images, sides multiples of 8 in [128,768] (seed %d), '
+                    '%d filters %dx%d, lambda log-uniform [0.005,0.1], rho 50, '
+                    '%d iterations' % (n_images, seed, M, K, K, iters),
+        'pixels': N,
+        'batched_s': tb, 'loop_s': tl, 'ratio': tl / tb,
+        'batched_Mpix_iters_per_s': N * iters / tb / 1e6,
+        'loop_Mpix_iters_per_s': N * iters / tl / 1e6,
+        'max_rel_residual_diff': worst, 'max_rel_Y_diff': ydiff,
+        'config': {k: cfg[k] for k in ('col_lds', 'row_lds', 'solve_lanes',
+                                       'solve_maps', 'persistent', 'graph')},
+        'device_solve_ms': cfg['solve_ms'], 'device_setup_ms': cfg['setup_ms'],
+        'first_call_tune_ms': cfg0['tune_ms'],
+    }
+
+
+def adjoint_leg(reps=3):
+    """Unrolled CBPDN forward vs forward + adjoint: 16 images 256x256 (clean
+    + N(0, 0.05^2)), 64 unit-norm 11x11 filters, lambda 0.03, rho 50, 50
+    iterations, checkpoints every 10."""
+    import numpy as np
+    import torch
+    from sporco_cuda import _abi
+    B, R, C, M, K, NIT, CK = 16, 256, 256, 64, 11, 50, 10
+    dev = torch.device('cuda:0')
+    rng = np.random.default_rng(4)
+    clean = np.stack([synth_image(R, C, 1004 + i) for i in range(B)])
+    noisy = clean + 0.05 * rng.standard_normal(clean.shape).astype(np.float32)
+    S = torch.from_numpy(noisy.astype(np.float32)).to(dev)
+    D = torch.from_numpy(synth_dict(K, M, 2004)).to(dev)
+    L = _abi.lib()
+    nck = L.scc_unrolled_ckpt_floats(M, B, R, C, NIT, CK)
+    ck = torch.empty(nck, dtype=torch.float32, device=dev)
+    Y = torch.empty((B, M, R, C), dtype=torch.float32, device=dev)
+    gY = torch.randn_like(Y)
+    gD = torch.empty_like(D)
+    gS = torch.empty_like(S)
+    gl = ctypes.c_float()
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def fwd():
+        rc = L.scc_unrolled_forward(D.data_ptr(), M, K, S.data_ptr(), B, R, C,
+                                    0.03, 50.0, 1.8, NIT, CK, Y.data_ptr(),
+                                    ck.data_ptr(), stream)
+        if rc:
+            raise RuntimeError(L.scc_unrolled_last_error().decode())
+
+    def bwd():
+        rc = L.scc_unrolled_backward(D.data_ptr(), M, K, S.data_ptr(), B, R,
+                                     C, 0.03, 50.0, 1.8, NIT, CK,
+                                     ck.data_ptr(), gY.data_ptr(),
+                                     gD.data_ptr(), gS.data_ptr(),
+                                     ctypes.addressof(gl), stream)
+        if rc:
+            raise RuntimeError(L.scc_unrolled_last_error().decode())
+    print('bench: adjoint leg', file=sys.stderr, flush=True)
+    fwd()
+    bwd()
+    torch.cuda.synchronize()
+    tf, tfb = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fwd()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        bwd()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        tf.append(t1 - t0)
+        tfb.append(t2 - t0)
+    f, fb = min(tf), min(tfb)
+    return {
+        'workload': '%d images %dx%d, %d filters %dx%d, lambda 0.03, rho 50, '
+                    '%d iterations, checkpoint every %d' % (B, R, C, M, K, K,
+                                                             NIT, CK),
+        'forward_ms': 1000 * f, 'forward_adjoint_ms': 1000 * fb,
+        'ratio': fb / f,
+    }
+
+
 def load_traffic(workload, stage):
     path = os.path.join(ROOT, 'profiles', 'traffic.json')
     if not os.path.exists(path):
@@ -143,6 +281,8 @@ def main():
                     choices=sorted(WORKLOADS))
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-batched', action='store_true')
+    ap.add_argument('--no-adjoint', action='store_true')
     args = ap.parse_args()
     if args.gpus != 1:
         sys.stderr.write('bench.py measures one GPU: this build runs the '
@@ -231,6 +371,8 @@ def main():
         'kernels': per_kernel,
         'setup_ms': info['setup_ms'],
     }
+    out['batched'] = None if args.no_batched else batched_leg()
+    out['adjoint'] = None if args.no_adjoint else adjoint_leg()
     if not args.no_cpu:
         print('bench: cpu baseline (%.0f s budget)' % args.cpu_seconds,
               file=sys.stderr, flush=True)

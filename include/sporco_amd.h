@@ -164,6 +164,19 @@ int scc_unrolled_tape(const float *D, int M, int K, const float *S, int B,
                       int n_iter, float *V, void *stream);
 const char *scc_unrolled_last_error(void);
 
+/* Reconstruction out_b = sum_m d_m * x_{b,m} (circular convolution with the
+ * filters zero-padded at the origin) of B coefficient sets, and its adjoint
+ * (gD = dL/dD (M, Kh, Kw), gX = dL/dX (B, M, R, C) from gout = dL/dout
+ * (B, R, C); either output may be NULL).  Device pointers; D (M, Kh, Kw),
+ * X (B, M, R, C), out (B, R, C).  No counterpart in the reference; it
+ * replaces sporco's fft.fftconv use in examples/big_cbpdn.py:69. */
+int scc_reconstruct(const float *D, int M, int Kh, int Kw, const float *X,
+                    int B, int R, int C, float *out, void *stream);
+int scc_reconstruct_backward(const float *D, int M, int Kh, int Kw,
+                             const float *X, int B, int R, int C,
+                             const float *gout, float *gD, float *gX,
+                             void *stream);
+
 /* Batched plain CBPDN: nimg images of any sizes sharing one dictionary D
  * (M, Kh, Kw), each with its own lambda, solved together.  Every ADMM stage
  * is one launch over a work list that spans all images, and the iteration

@@ -67,6 +67,36 @@ bool make_plan(int L, int tw_stride, FftPlan* pl) {
   return true;
 }
 
+int make_geom(int R, int C, int M, Geom* g, bool* pow2, Pow2Config* p2) {
+  g->R = R;
+  g->C = C;
+  g->M = M;
+  g->H = C / 2 + 1;
+  g->even = (C % 2 == 0) ? 1 : 0;
+  g->Lr = g->even ? C / 2 : C;
+  g->N = (long long)R * C;
+  g->Hc = (long long)R * g->H;
+  g->lgR = -1;
+  if ((R & (R - 1)) == 0)
+    for (g->lgR = 0; (1 << g->lgR) < R; ++g->lgR) {
+    }
+  *pow2 = pow2_select(R, C, M, p2);
+  int TC;
+  if (*pow2) {
+    TC = p2->TC;
+  } else {
+    int cap = 65536 / (2 * (R + 1) * 8);
+    if (cap < 1) cap = 1;
+    cap = std::min(cap, g->H);
+    TC = 1;
+    while (TC * 2 <= cap) TC *= 2;
+  }
+  g->TC = TC;
+  g->P = (g->H + TC - 1) / TC;
+  g->wlast = g->H - (g->P - 1) * TC;
+  return SCC_OK;
+}
+
 namespace {
 
 thread_local scc_info g_info;
@@ -141,36 +171,6 @@ struct Workspace {
 std::mutex g_mu;
 std::map<std::tuple<int, int, int, int>, std::unique_ptr<Workspace>> g_ws;
 
-int make_geom(int R, int C, int M, Geom* g, bool* pow2, Pow2Config* p2) {
-  g->R = R;
-  g->C = C;
-  g->M = M;
-  g->H = C / 2 + 1;
-  g->even = (C % 2 == 0) ? 1 : 0;
-  g->Lr = g->even ? C / 2 : C;
-  g->N = (long long)R * C;
-  g->Hc = (long long)R * g->H;
-  g->lgR = -1;
-  if ((R & (R - 1)) == 0)
-    for (g->lgR = 0; (1 << g->lgR) < R; ++g->lgR) {
-    }
-  *pow2 = pow2_select(R, C, M, p2);
-  int TC;
-  if (*pow2) {
-    TC = p2->TC;
-  } else {
-    int cap = 65536 / (2 * (R + 1) * 8);
-    if (cap < 1) cap = 1;
-    cap = std::min(cap, g->H);
-    TC = 1;
-    while (TC * 2 <= cap) TC *= 2;
-  }
-  g->TC = TC;
-  g->P = (g->H + TC - 1) / TC;
-  g->wlast = g->H - (g->P - 1) * TC;
-  return SCC_OK;
-}
-
 int get_workspace(int device, int R, int C, int M, Workspace** out) {
   auto key = std::make_tuple(device, R, C, M);
   auto it = g_ws.find(key);
@@ -204,8 +204,11 @@ int get_workspace(int device, int R, int C, int M, Workspace** out) {
       (rc = ensure(ws->twC, (size_t)C * 8)))
     return rc;
   const int nb = ws->pow2 ? pow2_row_blocks(g, ws->p2) : row_blocks(g);
+  // Block sums: 64 reduction blocks, or one per column block when the
+  // first reduction pass runs inside the power-of-two column kernel.
+  const size_t nbs = std::max<size_t>(64, ws->pow2 ? (size_t)g.P * M : 0);
   if ((rc = ensure(ws->partials, (size_t)5 * nb * 4)) ||
-      (rc = ensure(ws->blocksum, (size_t)5 * 64 * 8)))
+      (rc = ensure(ws->blocksum, 5 * nbs * 8)))
     return rc;
   SCC_HIP(hipStreamCreateWithFlags(&ws->stream, hipStreamNonBlocking));
   for (auto& e : ws->ev) SCC_HIP(hipEventCreate(&e));
@@ -468,14 +471,26 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
   const bool can_stop = o.RelStopTol > 0.f || o.AbsStopTol > 0.f;
   const int poll = o.Verbose == 1 ? 1 : 16;
   int nblocks = 0;
+  // Power-of-two path: the residual reduction of an iteration that is not
+  // an AutoRho step, a host poll or the last one is deferred into the next
+  // iteration (first pass in the column kernel's blocks, second pass and
+  // control update in block 0 of the solve, before the inverse transforms
+  // and the row kernel test the stopping flag), saving two launches.
+  bool pending = false;
   for (int i = 1; i <= maxit; ++i) {
     const bool rho_step = p.autorho && i > 1 && i % p.period == 0;
     const int rmode = rho_step ? kRowUpdate : kRowFused;
+    const bool poll_pt =
+        (can_stop || o.Verbose == 1) && (i % poll == 0 || i == maxit);
+    const bool defer = ws->pow2 && !rho_step && !poll_pt && i < maxit;
     SCC_HIP(mark(-1));
     if (ws->pow2) {
-      SCC_HIP(pow2_col(g, ws->p2, M, b.F, b.twR, 0, b.ctrl, st));
+      SCC_HIP(pow2_col(g, ws->p2, M, b.F, b.twR, 0, b.ctrl, st,
+                       pending ? b.partials : nullptr, pending ? nblocks : 0,
+                       b.blocksum));
       SCC_HIP(mark(kStColFwd));
-      SCC_HIP(pow2_solve(g, ws->p2, b, p, st));
+      SCC_HIP(pow2_solve(g, ws->p2, b, p, st, 1, pending ? i - 1 : 0,
+                         pending ? g.P * M : 0));
       SCC_HIP(mark(kStSolve));
       SCC_HIP(pow2_col(g, ws->p2, M, b.F, b.twR, 1, b.ctrl, st));
       SCC_HIP(mark(kStColInv));
@@ -490,8 +505,11 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
       SCC_HIP(launch_row(g, ws->rowp, b, p, rmode, &nblocks, st));
     }
     SCC_HIP(mark(kStRow));
-    SCC_HIP(launch_reduce(b, p, nblocks, i, maxit, st));
-    SCC_HIP(mark(kStReduce));
+    pending = defer;
+    if (!defer) {
+      SCC_HIP(launch_reduce(b, p, nblocks, i, maxit, st));
+      SCC_HIP(mark(kStReduce));
+    }
     if (rho_step) {
       if (ws->pow2)
         SCC_HIP(pow2_row_fwd(g, ws->p2, M, kFwdYU, nullptr, b.Y, b.U, b.F,

@@ -297,4 +297,31 @@ __device__ inline void row_body(const Geom& g, const FftPlan& rp,
   }
 }
 
+// Inverse real row transform of rows blk*nr.. of the half spectra F into
+// out (real rows of length C), times scale.
+__device__ inline void row_inv_body(const Geom& g, long long nrows,
+                                    const FftPlan& rp, const float2* F,
+                                    float* out, const float2* __restrict__ twC,
+                                    float scale, int nr, long long blk,
+                                    float2* lds) {
+  const int ld = g.Lr + 1;
+  float2* A = lds;
+  float2* B = lds + nr * ld;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const long long row0 = blk * nr;
+  const int nt = (int)min((long long)nr, nrows - row0);
+  spec_load(g, F, B, nt, ld, row0, tid, nthr);
+  __syncthreads();
+  c2r_prep(g, B, A, nt, ld, twC, tid, nthr);
+  __syncthreads();
+  const float2* P = fft_generic(A, B, nt, ld, rp, twC, -1.f, tid, nthr);
+  for (int i = tid; i < nt * g.C; i += nthr) {
+    const int t = i / g.C;
+    const int n = i - t * g.C;
+    const float xv = g.even ? reinterpret_cast<const float*>(P + t * ld)[n]
+                            : P[t * ld + n].x;
+    out[(row0 + t) * g.C + n] = xv * scale;
+  }
+}
+
 }  // namespace scc

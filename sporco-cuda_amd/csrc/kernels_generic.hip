@@ -50,6 +50,14 @@ __global__ __launch_bounds__(256) void k_row_fwd_generic(
                lds);
 }
 
+// Inverse real row transform of half spectra into real rows, times scale.
+__global__ __launch_bounds__(256) void k_row_inv_generic(
+    Geom g, long long nrows, FftPlan rp, const float2* F, float* out,
+    const float2* __restrict__ twC, float scale, int nr) {
+  extern __shared__ float2 lds[];
+  row_inv_body(g, nrows, rp, F, out, twC, scale, nr, blockIdx.x, lds);
+}
+
 // --------------------------------------------------------------------------
 // Column transforms, one panel of one map per block, in place.
 // --------------------------------------------------------------------------
@@ -203,6 +211,18 @@ hipError_t launch_row_fwd(const Geom& g, int nmaps, const FftPlan& rp,
   const size_t shm = (size_t)2 * nr * (g.Lr + 1) * sizeof(float2);
   hipLaunchKernelGGL(k_row_fwd_generic, dim3(nb), dim3(256), shm, st, g,
                      nrows, rp, mode, src, Y, U, F, twC, ctrl, nr);
+  return hipGetLastError();
+}
+
+hipError_t launch_row_inv(const Geom& g, int nmaps, const FftPlan& rp,
+                          float2* F, float* out, const float2* twC,
+                          float scale, hipStream_t st) {
+  const int nr = row_nr(g);
+  const long long nrows = (long long)nmaps * g.R;
+  const unsigned nb = (unsigned)((nrows + nr - 1) / nr);
+  const size_t shm = (size_t)2 * nr * (g.Lr + 1) * sizeof(float2);
+  hipLaunchKernelGGL(k_row_inv_generic, dim3(nb), dim3(256), shm, st, g,
+                     nrows, rp, F, out, twC, scale, nr);
   return hipGetLastError();
 }
 

@@ -49,15 +49,48 @@ __device__ __forceinline__ void upd1(const Params& p, float x, float yo,
   acc.s += dy * dy;
 }
 
+// First pass of a deferred residual reduction, run by the blocks of the
+// next column kernel: block b of nb adds its slice of the per-wavefront
+// partial sums (5 arrays of nparts floats) in double and stores the five
+// results to bsum[5 b + j].  Fixed order, so the sums are deterministic.
+__device__ __forceinline__ void partials_slice(const float* __restrict__ parts,
+                                               int nparts, double* bsum,
+                                               int b, int nb,
+                                               double (*red)[16]) {
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int chunk = (nparts + nb - 1) / nb;
+  const int i0 = b * chunk;
+  const int i1 = min(nparts, i0 + chunk);
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int i = i0 + tid; i < i1; i += nthr)
+#pragma unroll
+    for (int j = 0; j < 5; ++j) acc[j] += (double)parts[j * nparts + i];
+  const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    double v = acc[j];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if (lane == 0) red[j][wv] = v;
+  }
+  __syncthreads();
+  if (tid < 5) {
+    double v = 0.0;
+    for (int w = 0; w < (nthr >> 6); ++w) v += red[tid][w];
+    bsum[5 * b + tid] = v;
+  }
+}
+
 }  // namespace
 
 // --------------------------------------------------------------------------
-// Column transforms
+// Column transforms.  With nparts > 0 the blocks also run the first pass
+// of the previous row kernel's residual reduction (partials_slice).
 // --------------------------------------------------------------------------
 template <int R, int TC, int DIR>
 __global__ __launch_bounds__(TC*(R / kColE)) void k_col_pow2(
     Geom g, float2* __restrict__ F, const float2* __restrict__ twR,
-    const Ctrl* ctrl) {
+    const Ctrl* ctrl, const float* __restrict__ parts, int nparts,
+    double* bsum) {
   constexpr int E = kColE;
   constexpr int T = R / E;
   // Column stride congruent to 32 / TC (mod 32) in 8-byte words: every
@@ -66,7 +99,11 @@ __global__ __launch_bounds__(TC*(R / kColE)) void k_col_pow2(
   constexpr int L0 = reg::lds_len(R);
   constexpr int LP = L0 + ((32 / TC - L0 % 32) + 32) % 32;
   __shared__ float2 lds[TC * LP];
+  __shared__ double red[5][16];
   if (ctrl != nullptr && ctrl->done) return;
+  if (nparts > 0)
+    partials_slice(parts, nparts, bsum, blockIdx.y * gridDim.x + blockIdx.x,
+                   gridDim.x * gridDim.y, red);
   const int p = blockIdx.x;
   const long long m = blockIdx.y;
   const int w = (p == g.P - 1) ? g.wlast : TC;
@@ -93,9 +130,37 @@ __global__ __launch_bounds__(TC*(R / kColE)) void k_col_pow2(
 // --------------------------------------------------------------------------
 // Sherman-Morrison X-solve: G lanes per frequency, MPT maps per lane.
 // --------------------------------------------------------------------------
+// With ctl_iter > 0, block 0 first completes the deferred residual
+// reduction of iteration ctl_iter (nbsum block sums from the column kernel,
+// added in a fixed order) and runs the control update; that iteration was
+// not an AutoRho step, so rho is unchanged for the other blocks.
 template <int G, int MPT, bool GRAD>
 __global__ __launch_bounds__(256) void k_solve_pow2(Geom g, Buffers b,
-                                                    Params p) {
+                                                    Params p, int ctl_iter,
+                                                    int nbsum) {
+  if (ctl_iter > 0 && blockIdx.x == 0 && blockIdx.y == 0 && !b.ctrl->done) {
+    __shared__ double red[5][4];
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < nbsum; i += 256)
+#pragma unroll
+      for (int j = 0; j < 5; ++j) acc[j] += b.blocksum[5 * i + j];
+    const int ln = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      double v = acc[j];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+      if (ln == 0) red[j][w] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double s5[5];
+#pragma unroll
+      for (int j = 0; j < 5; ++j)
+        s5[j] = ((red[j][0] + red[j][1]) + red[j][2]) + red[j][3];
+      control_update(b, p, s5, ctl_iter);
+    }
+    __syncthreads();
+  }
   if (b.ctrl->done) return;
   constexpr int FPW = 64 / G;
   const int lane = threadIdx.x & 63;
@@ -379,6 +444,40 @@ int col_tc(int R) {
   return R <= 512 ? 16 : (R == 1024 ? 8 : 4);
 }
 
+// Inverse real row transform of half spectra (panel layout) into real
+// rows, times scale: one or more rows per wavefront as in the row kernel.
+template <int LR, int TC>
+__global__ __launch_bounds__(256) void k_row_inv_pow2(Geom g,
+                                                      const float2* F,
+                                                      float* out,
+                                                      const float2* twC,
+                                                      float scale) {
+  using RC = RowCfg<LR>;
+  constexpr int E = RC::E, T = RC::T, RPW = RC::RPW;
+  constexpr int C = 2 * LR;
+  constexpr int LP = reg::lds_len(LR);
+  __shared__ float2 lds[4 * RPW * LP];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int rw = lane / T;
+  const int t = lane & (T - 1);
+  const int partner = rw * T + ((T - t) & (T - 1));
+  const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wv);
+  const int row0 = wave * RPW;
+  const int R = g.R;
+  const int m = row0 >> g.lgR;
+  const int r = (row0 & (R - 1)) + rw;
+  float2* L = lds + (wv * RPW + rw) * LP;
+  float2 z[E];
+  load_spectrum_c2r<LR, TC>(z, t, partner, r, R, F + (long long)m * g.Hc,
+                            twC);
+  reg::fft_wave<LR, E, -1>(z, t, L, twC, 2);
+  float2* O2 = reinterpret_cast<float2*>(out + (long long)row0 * C);
+  const int lo = rw * LR + t;
+#pragma unroll
+  for (int e = 0; e < E; ++e) O2[lo + T * e] = cscale(z[e], scale);
+}
+
 template <int LR, int TC>
 hipError_t row_launch(const Geom& g, const Buffers& b, const Params& p,
                       int mode, int* nblocks, hipStream_t st) {
@@ -452,19 +551,37 @@ hipError_t pow2_row_fwd(const Geom& g, const Pow2Config& cfg, int nmaps,
   return hipErrorNotSupported;
 }
 
+hipError_t pow2_row_inv(const Geom& g, const Pow2Config& cfg, int nmaps,
+                        const float2* F, float* out, const float2* twC,
+                        float scale, hipStream_t st) {
+  const int LR = g.C / 2;
+#define X(lr, tc)                                                           \
+  if (LR == lr && cfg.TC == tc) {                                           \
+    const unsigned nb =                                                     \
+        (unsigned)((long long)nmaps * g.R / RowCfg<lr>::RPB);               \
+    hipLaunchKernelGGL((k_row_inv_pow2<lr, tc>), dim3(nb), dim3(256), 0,    \
+                       st, g, F, out, twC, scale);                          \
+    return hipGetLastError();                                               \
+  }
+  SCC_ROW_CASES(X)
+#undef X
+  return hipErrorNotSupported;
+}
+
 hipError_t pow2_col(const Geom& g, const Pow2Config& cfg, int nmaps,
                     float2* F, const float2* twR, int inverse,
-                    const Ctrl* ctrl, hipStream_t st) {
+                    const Ctrl* ctrl, hipStream_t st, const float* parts,
+                    int nparts, double* bsum) {
   const dim3 grid(g.P, nmaps);
 #define X(r, tc)                                                          \
   if (g.R == r && cfg.TC == tc) {                                         \
     const dim3 blk(tc * (r / kColE));                                     \
     if (inverse)                                                          \
       hipLaunchKernelGGL((k_col_pow2<r, tc, -1>), grid, blk, 0, st, g, F, \
-                         twR, ctrl);                                      \
+                         twR, ctrl, parts, nparts, bsum);                 \
     else                                                                  \
       hipLaunchKernelGGL((k_col_pow2<r, tc, 1>), grid, blk, 0, st, g, F,  \
-                         twR, ctrl);                                      \
+                         twR, ctrl, parts, nparts, bsum);                 \
     return hipGetLastError();                                             \
   }
   X(256, 16) X(512, 16) X(1024, 8) X(2048, 4)
@@ -473,7 +590,8 @@ hipError_t pow2_col(const Geom& g, const Pow2Config& cfg, int nmaps,
 }
 
 hipError_t pow2_solve(const Geom& g, const Pow2Config& cfg, const Buffers& b,
-                      const Params& p, hipStream_t st, int nimg) {
+                      const Params& p, hipStream_t st, int nimg, int ctl_iter,
+                      int nbsum) {
   const int G = cfg.variant;
   const long long fpb = 4LL * (64 / G);  // frequencies per 256-thread block
   const dim3 nb((unsigned)((g.Hc + fpb - 1) / fpb), (unsigned)nimg);
@@ -481,10 +599,10 @@ hipError_t pow2_solve(const Geom& g, const Pow2Config& cfg, const Buffers& b,
   if (G == gg) {                                                           \
     if (p.grad)                                                            \
       hipLaunchKernelGGL((k_solve_pow2<gg, 16, true>), dim3(nb), dim3(256), \
-                         0, st, g, b, p);                                  \
+                         0, st, g, b, p, ctl_iter, nbsum);                 \
     else                                                                   \
       hipLaunchKernelGGL((k_solve_pow2<gg, 16, false>), dim3(nb),           \
-                         dim3(256), 0, st, g, b, p);                       \
+                         dim3(256), 0, st, g, b, p, ctl_iter, nbsum);      \
     return hipGetLastError();                                              \
   }
   X(1) X(2) X(4) X(8) X(16)

@@ -99,19 +99,4 @@ __device__ __forceinline__ void load_spectrum_c2r(float2 (&z)[RowCfg<LR>::E],
   }
 }
 
-// Column index of flat half-spectrum offset f (< Hc) in the panel layout.
-__device__ __forceinline__ int panel_col(const Geom& g, long long f) {
-  const int pr = g.TC * g.R;
-  const int p = (int)(f / pr);
-  const int o = (int)(f - (long long)p * pr);
-  const int w = (p == g.P - 1) ? g.wlast : g.TC;
-  return p * g.TC + o % w;
-}
-
-// Multiplicity of a half-spectrum column in the full spectrum: 1 for DC
-// (and Nyquist when C is even), 2 for the others.
-__device__ __forceinline__ float col_mult(const Geom& g, int c) {
-  return (c == 0 || (g.even && c == g.C / 2)) ? 1.f : 2.f;
-}
-
 }  // namespace scc

@@ -130,5 +130,14 @@ int row_blocks(const Geom& g);
 // returned by scc_last_error, and the mixed-radix plan of one FFT length.
 int set_error(int code, const std::string& msg);
 bool make_plan(int L, int tw_stride, FftPlan* pl);
+struct Pow2Config;
+// Geometry of an (R, C) image with M maps; *pow2 tells whether the
+// register-resident power-of-two kernels apply (their config in *p2).
+int make_geom(int R, int C, int M, Geom* g, bool* pow2, Pow2Config* p2);
+// Inverse real row transforms of nmaps maps of half spectra F (panel
+// layout, destroyed) into out (nmaps, R, C), times scale.
+hipError_t launch_row_inv(const Geom& g, int nmaps, const FftPlan& rp,
+                          float2* F, float* out, const float2* twC,
+                          float scale, hipStream_t st);
 
 }  // namespace scc

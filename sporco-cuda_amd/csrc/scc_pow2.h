@@ -19,13 +19,26 @@ hipError_t pow2_row_fwd(const Geom& g, const Pow2Config& cfg, int nmaps,
                         int mode, const float* src, float* Y, float* U,
                         float2* F, const float2* twC, const Ctrl* ctrl,
                         hipStream_t st);
+// Inverse real row transforms of nmaps maps of half spectra into real
+// (nmaps, R, C) rows, times scale.
+hipError_t pow2_row_inv(const Geom& g, const Pow2Config& cfg, int nmaps,
+                        const float2* F, float* out, const float2* twC,
+                        float scale, hipStream_t st);
+// With nparts > 0 the column kernel also sums slices of the previous row
+// kernel's nparts residual partials into bsum (5 doubles per block, P x
+// nmaps blocks): the first pass of a deferred reduction.
 hipError_t pow2_col(const Geom& g, const Pow2Config& cfg, int nmaps,
                     float2* F, const float2* twR, int inverse,
-                    const Ctrl* ctrl, hipStream_t st);
+                    const Ctrl* ctrl, hipStream_t st,
+                    const float* parts = nullptr, int nparts = 0,
+                    double* bsum = nullptr);
 // nimg same-size images (maps of image i at F + i M Hc, spectrum at
-// Sf + i Hc) share the dictionary spectra.
+// Sf + i Hc) share the dictionary spectra.  ctl_iter > 0: block 0 first
+// finishes the deferred reduction of iteration ctl_iter from the nbsum
+// block sums in b.blocksum and runs the control update.
 hipError_t pow2_solve(const Geom& g, const Pow2Config& cfg, const Buffers& b,
-                      const Params& p, hipStream_t st, int nimg = 1);
+                      const Params& p, hipStream_t st, int nimg = 1,
+                      int ctl_iter = 0, int nbsum = 0);
 hipError_t pow2_row(const Geom& g, const Pow2Config& cfg, const Buffers& b,
                     const Params& p, int mode, int* nblocks, hipStream_t st);
 

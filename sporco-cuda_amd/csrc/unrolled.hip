@@ -199,12 +199,11 @@ __global__ __launch_bounds__(256) void k_adj_solve(
   }
 }
 
-// Unnormalised inverse real transform of every row's half spectrum:
-// MODE 0 adds it to Ybar and subtracts it from Ubar, MODE 1 stores it.
-template <int LR, int TC, int MODE>
+// Unnormalised inverse real transform of every row's half spectrum, added
+// to Ybar and subtracted from Ubar.
+template <int LR, int TC>
 __global__ __launch_bounds__(256) void k_adj_row2(Geom g, const float2* F,
                                                   float* Yb, float* Ub,
-                                                  float* out,
                                                   const float2* twC) {
   using RC = RowCfg<LR>;
   constexpr int E = RC::E, T = RC::T, RPW = RC::RPW;
@@ -228,19 +227,13 @@ __global__ __launch_bounds__(256) void k_adj_row2(Geom g, const float2* F,
   load_spectrum_c2r<LR, TC>(z, t, partner, r, R, F + (long long)m * g.Hc,
                             twC);
   reg::fft_wave<LR, E, -1>(z, t, L, twC, 2);
-  if (MODE == 0) {
-    float2* Y2 = reinterpret_cast<float2*>(Yb + base);
-    float2* U2 = reinterpret_cast<float2*>(Ub + base);
+  float2* Y2 = reinterpret_cast<float2*>(Yb + base);
+  float2* U2 = reinterpret_cast<float2*>(Ub + base);
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const float2 y = Y2[lo + T * e], u = U2[lo + T * e];
-      Y2[lo + T * e] = cadd(y, z[e]);
-      U2[lo + T * e] = csub(u, z[e]);
-    }
-  } else {
-    float2* O2 = reinterpret_cast<float2*>(out + base);
-#pragma unroll
-    for (int e = 0; e < E; ++e) O2[lo + T * e] = z[e];
+  for (int e = 0; e < E; ++e) {
+    const float2 y = Y2[lo + T * e], u = U2[lo + T * e];
+    Y2[lo + T * e] = cadd(y, z[e]);
+    U2[lo + T * e] = csub(u, z[e]);
   }
 }
 
@@ -306,20 +299,18 @@ hipError_t adj_row1(const Geom& gbm, const Pow2Config& cfg, float* Yb,
   return hipErrorNotSupported;
 }
 
-hipError_t adj_row2(const Geom& gbm, const Pow2Config& cfg, const float2* F,
+// mode 0: Ybar += irfft, Ubar -= irfft; mode 1: out = irfft (unscaled).
+hipError_t adj_row2(const Geom& gbm, const Pow2Config& cfg, float2* F,
                     float* Yb, float* Ub, float* out, const float2* twC,
                     int mode, hipStream_t st) {
+  if (mode == 1) return pow2_row_inv(gbm, cfg, gbm.M, F, out, twC, 1.f, st);
   const int LR = gbm.C / 2;
 #define X(lr, tc)                                                           \
   if (LR == lr && cfg.TC == tc) {                                           \
     const unsigned nb = (unsigned)((long long)gbm.M * gbm.R /               \
                                    RowCfg<lr>::RPB);                         \
-    if (mode == 0)                                                          \
-      hipLaunchKernelGGL((k_adj_row2<lr, tc, 0>), dim3(nb), dim3(256), 0,   \
-                         st, gbm, F, Yb, Ub, out, twC);                     \
-    else                                                                    \
-      hipLaunchKernelGGL((k_adj_row2<lr, tc, 1>), dim3(nb), dim3(256), 0,   \
-                         st, gbm, F, Yb, Ub, out, twC);                     \
+    hipLaunchKernelGGL((k_adj_row2<lr, tc>), dim3(nb), dim3(256), 0, st,    \
+                       gbm, F, Yb, Ub, twC);                                \
     return hipGetLastError();                                               \
   }
   SCC_UNR_CASES(X)

@@ -104,6 +104,9 @@ def lib():
                                       _i, _i, vp, vp, vp]),
         'scc_unrolled_backward': (_i, [vp, _i, _i, vp, _i, _i, _i, _f, _f,
                                        _f, _i, _i, vp, vp, vp, vp, vp, vp]),
+        'scc_reconstruct': (_i, [vp, _i, _i, _i, vp, _i, _i, _i, vp, vp]),
+        'scc_reconstruct_backward': (_i, [vp, _i, _i, _i, vp, _i, _i, _i, vp,
+                                          vp, vp, vp]),
         'scc_batched_cbpdn': (_i, [vp, _i, _i, _i, _i, vp, vp, vp, vp]),
         'scc_batched_set_autotune': (_i, [_i]),
         'scc_batched_set_graph': (_i, [_i]),

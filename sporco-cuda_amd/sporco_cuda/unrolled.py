@@ -83,6 +83,59 @@ class CBPDNUnrolled(torch.autograd.Function):
         return gD, gS, glam, None, None, None, None
 
 
+def _rc_check(rc):
+    if rc != 0:
+        msg = _abi.last_error()
+        if rc in (_abi.SCC_ERR_INVALID, _abi.SCC_ERR_UNSUPPORTED):
+            raise ValueError(msg)
+        if rc == _abi.SCC_ERR_NOMEM:
+            raise MemoryError(msg)
+        raise RuntimeError('sporco_cuda error %d: %s' % (rc, msg))
+
+
+class Reconstruct(torch.autograd.Function):
+    """out (B, R, C) = sum_m d_m * x_{b,m} for D (M, K, K), X (B, M, R, C),
+    through scc_reconstruct / scc_reconstruct_backward."""
+
+    @staticmethod
+    def forward(ctx, D, X):
+        if D.dtype != torch.float32 or X.dtype != torch.float32:
+            raise ValueError('D and X must be float32')
+        if not (D.is_cuda and X.is_cuda):
+            raise ValueError('D and X must be on the GPU')
+        D = D.contiguous()
+        X = X.contiguous()
+        M, Kh, Kw = D.shape
+        B, M2, R, C = X.shape
+        if M2 != M:
+            raise ValueError('X must hold one map per filter')
+        out = torch.empty((B, R, C), dtype=torch.float32, device=X.device)
+        _rc_check(_abi.lib().scc_reconstruct(
+            D.data_ptr(), M, Kh, Kw, X.data_ptr(), B, R, C, out.data_ptr(),
+            _stream(D)))
+        ctx.save_for_backward(D, X)
+        return out
+
+    @staticmethod
+    def backward(ctx, gout):
+        D, X = ctx.saved_tensors
+        M, Kh, Kw = D.shape
+        B, _, R, C = X.shape
+        gout = gout.contiguous()
+        gD = torch.empty_like(D) if ctx.needs_input_grad[0] else None
+        gX = torch.empty_like(X) if ctx.needs_input_grad[1] else None
+        _rc_check(_abi.lib().scc_reconstruct_backward(
+            D.data_ptr(), M, Kh, Kw, X.data_ptr(), B, R, C, gout.data_ptr(),
+            None if gD is None else gD.data_ptr(),
+            None if gX is None else gX.data_ptr(), _stream(D)))
+        return gD, gX
+
+
+def reconstruct(D, X):
+    """sum_m d_m * x_m per image (differentiable in D and X)."""
+    return Reconstruct.apply(D, X)
+
+
 def cbpdn_unrolled(D, S, lmbda, rho=50.0, n_iter=50, relax=1.8,
                    ckpt_every=10):
     """Y (B, M, R, C) after n_iter unrolled ADMM iterations (differentiable
