@@ -204,11 +204,10 @@ int get_workspace(int device, int R, int C, int M, Workspace** out) {
       (rc = ensure(ws->twC, (size_t)C * 8)))
     return rc;
   const int nb = ws->pow2 ? pow2_row_blocks(g, ws->p2) : row_blocks(g);
-  // Block sums: 64 reduction blocks, or one per column block when the
-  // first reduction pass runs inside the power-of-two column kernel.
-  const size_t nbs = std::max<size_t>(64, ws->pow2 ? (size_t)g.P * M : 0);
+  // Block sums of the first reduction pass (at most 64 blocks, in the
+  // reduction kernel or in the power-of-two column kernel).
   if ((rc = ensure(ws->partials, (size_t)5 * nb * 4)) ||
-      (rc = ensure(ws->blocksum, 5 * nbs * 8)))
+      (rc = ensure(ws->blocksum, (size_t)5 * 64 * 8)))
     return rc;
   SCC_HIP(hipStreamCreateWithFlags(&ws->stream, hipStreamNonBlocking));
   for (auto& e : ws->ev) SCC_HIP(hipEventCreate(&e));
@@ -490,7 +489,7 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
                        b.blocksum));
       SCC_HIP(mark(kStColFwd));
       SCC_HIP(pow2_solve(g, ws->p2, b, p, st, 1, pending ? i - 1 : 0,
-                         pending ? g.P * M : 0));
+                         pending ? pow2_defer_blocks(g, M) : 0));
       SCC_HIP(mark(kStSolve));
       SCC_HIP(pow2_col(g, ws->p2, M, b.F, b.twR, 1, b.ctrl, st));
       SCC_HIP(mark(kStColInv));

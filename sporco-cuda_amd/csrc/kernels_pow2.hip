@@ -83,14 +83,16 @@ __device__ __forceinline__ void partials_slice(const float* __restrict__ parts,
 }  // namespace
 
 // --------------------------------------------------------------------------
-// Column transforms.  With nparts > 0 the blocks also run the first pass
-// of the previous row kernel's residual reduction (partials_slice).
+// Column transforms: one panel (TC columns x R rows, row-major, contiguous
+// in HBM) per block, E = 16 points per thread.  With nparts > 0 the first
+// nred blocks (in dispatch order) also run the first pass of the previous
+// row kernel's residual reduction (partials_slice) after their own stores.
 // --------------------------------------------------------------------------
 template <int R, int TC, int DIR>
 __global__ __launch_bounds__(TC*(R / kColE)) void k_col_pow2(
     Geom g, float2* __restrict__ F, const float2* __restrict__ twR,
     const Ctrl* ctrl, const float* __restrict__ parts, int nparts,
-    double* bsum) {
+    double* bsum, int nred) {
   constexpr int E = kColE;
   constexpr int T = R / E;
   // Column stride congruent to 32 / TC (mod 32) in 8-byte words: every
@@ -101,9 +103,6 @@ __global__ __launch_bounds__(TC*(R / kColE)) void k_col_pow2(
   __shared__ float2 lds[TC * LP];
   __shared__ double red[5][16];
   if (ctrl != nullptr && ctrl->done) return;
-  if (nparts > 0)
-    partials_slice(parts, nparts, bsum, blockIdx.y * gridDim.x + blockIdx.x,
-                   gridDim.x * gridDim.y, red);
   const int p = blockIdx.x;
   const long long m = blockIdx.y;
   const int w = (p == g.P - 1) ? g.wlast : TC;
@@ -125,6 +124,9 @@ __global__ __launch_bounds__(TC*(R / kColE)) void k_col_pow2(
 #pragma unroll
     for (int e = 0; e < E; ++e) base[(t + T * e) * w + c] = v[e];
   }
+  const int lin = blockIdx.y * gridDim.x + blockIdx.x;
+  if (nparts > 0 && lin < nred)
+    partials_slice(parts, nparts, bsum, lin, nred, red);
 }
 
 // --------------------------------------------------------------------------
@@ -440,9 +442,9 @@ namespace {
 
 bool is_pow2(int x) { return x > 0 && (x & (x - 1)) == 0; }
 
-int col_tc(int R) {
-  return R <= 512 ? 16 : (R == 1024 ? 8 : 4);
-}
+// Panel width: 16 columns up to R = 512; for longer columns fewer, so that
+// a column block keeps 512 threads and two blocks fit in one CU's LDS.
+int col_tc(int R) { return R <= 512 ? 16 : (R == 1024 ? 8 : 4); }
 
 // Inverse real row transform of half spectra (panel layout) into real
 // rows, times scale: one or more rows per wavefront as in the row kernel.
@@ -522,6 +524,10 @@ bool pow2_select(int R, int C, int M, Pow2Config* cfg) {
   return true;
 }
 
+int pow2_defer_blocks(const Geom& g, int nmaps) {
+  return std::min(64, g.P * nmaps);
+}
+
 int pow2_row_blocks(const Geom& g, const Pow2Config& cfg) {
   return (int)((long long)g.M * g.R / cfg.rows);  // one partial per wavefront
 }
@@ -573,15 +579,16 @@ hipError_t pow2_col(const Geom& g, const Pow2Config& cfg, int nmaps,
                     const Ctrl* ctrl, hipStream_t st, const float* parts,
                     int nparts, double* bsum) {
   const dim3 grid(g.P, nmaps);
+  const int nred = pow2_defer_blocks(g, nmaps);
 #define X(r, tc)                                                          \
   if (g.R == r && cfg.TC == tc) {                                         \
     const dim3 blk(tc * (r / kColE));                                     \
     if (inverse)                                                          \
       hipLaunchKernelGGL((k_col_pow2<r, tc, -1>), grid, blk, 0, st, g, F, \
-                         twR, ctrl, parts, nparts, bsum);                 \
+                         twR, ctrl, parts, nparts, bsum, nred);           \
     else                                                                  \
       hipLaunchKernelGGL((k_col_pow2<r, tc, 1>), grid, blk, 0, st, g, F,  \
-                         twR, ctrl, parts, nparts, bsum);                 \
+                         twR, ctrl, parts, nparts, bsum, nred);           \
     return hipGetLastError();                                             \
   }
   X(256, 16) X(512, 16) X(1024, 8) X(2048, 4)

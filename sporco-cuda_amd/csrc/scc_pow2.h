@@ -24,9 +24,11 @@ hipError_t pow2_row_fwd(const Geom& g, const Pow2Config& cfg, int nmaps,
 hipError_t pow2_row_inv(const Geom& g, const Pow2Config& cfg, int nmaps,
                         const float2* F, float* out, const float2* twC,
                         float scale, hipStream_t st);
-// With nparts > 0 the column kernel also sums slices of the previous row
-// kernel's nparts residual partials into bsum (5 doubles per block, P x
-// nmaps blocks): the first pass of a deferred reduction.
+// With nparts > 0 the first pow2_defer_blocks(g, nmaps) blocks of the
+// column kernel also sum slices of the previous row kernel's nparts
+// residual partials into bsum (5 doubles each): the first pass of a
+// deferred reduction.
+int pow2_defer_blocks(const Geom& g, int nmaps);
 hipError_t pow2_col(const Geom& g, const Pow2Config& cfg, int nmaps,
                     float2* F, const float2* twR, int inverse,
                     const Ctrl* ctrl, hipStream_t st,

@@ -243,3 +243,18 @@ def test_batched_no_device_is_a_status():
         pytest.skip('a device is present')
     rc, _ = _batched_call()
     assert rc == _abi.SCC_ERR_NODEVICE
+
+
+def test_one_hip_runtime_whatever_the_import_order():
+    """Importing the package before torch must not map a second HIP
+    runtime (two runtimes in one process cannot both open the device)."""
+    import subprocess
+    import sys
+    pytest.importorskip('torch')
+    code = ('import sys; sys.path.insert(0, %r); import sporco_cuda; '
+            'import torch; '
+            'maps = open("/proc/self/maps").read().split("\\n"); '
+            'print(len({l.split()[-1] for l in maps '
+            'if "libamdhip64" in l}))' % os.path.join(ROOT, 'sporco-cuda_amd'))
+    out = subprocess.check_output([sys.executable, '-c', code], text=True)
+    assert out.strip().splitlines()[-1] == '1', out
