@@ -237,7 +237,9 @@ __global__ __launch_bounds__(256) void k_solve_pow2(Geom g, Buffers b,
 // order of the real row as float2 pairs, so the update and the forward
 // transform of Y - U need no LDS pass at all.
 // --------------------------------------------------------------------------
-template <int LR, int TC, bool GENERAL>
+// TAPE (plain update only): also store V of every coefficient to b.vout,
+// the unrolled solver's tape.
+template <int LR, int TC, bool GENERAL, bool TAPE>
 __global__ __launch_bounds__(256) __attribute__((
     amdgpu_waves_per_eu(GENERAL ? 1 : RowCfg<LR>::WAVES))) void
 k_row_pow2(Geom g, Buffers b, Params p, int mode) {
@@ -357,7 +359,7 @@ k_row_pow2(Geom g, Buffers b, Params p, int mode) {
       const float yn0 = copysignf(fmaxf(fabsf(v0) - th, 0.f), v0);
       const float yn1 = copysignf(fmaxf(fabsf(v1) - th, 0.f), v1);
       const float un0 = v0 - yn0, un1 = v1 - yn1;
-      if (b.vout != nullptr)
+      if (TAPE)
         reinterpret_cast<float2*>(b.vout + (long long)row0 * C)[lo + T * e] =
             make_float2(v0, v1);
       acc.x = fmaf(x0, x0, fmaf(x1, x1, acc.x));
@@ -486,11 +488,14 @@ hipError_t row_launch(const Geom& g, const Buffers& b, const Params& p,
   const unsigned nb = (unsigned)((long long)g.M * g.R / RowCfg<LR>::RPB);
   const bool general = p.m0 || p.l1_mode == 2 || p.nonneg || p.nobndry;
   if (general)
-    hipLaunchKernelGGL((k_row_pow2<LR, TC, true>), dim3(nb), dim3(256), 0, st,
-                       g, b, p, mode);
+    hipLaunchKernelGGL((k_row_pow2<LR, TC, true, false>), dim3(nb), dim3(256),
+                       0, st, g, b, p, mode);
+  else if (b.vout != nullptr)
+    hipLaunchKernelGGL((k_row_pow2<LR, TC, false, true>), dim3(nb),
+                       dim3(256), 0, st, g, b, p, mode);
   else
-    hipLaunchKernelGGL((k_row_pow2<LR, TC, false>), dim3(nb), dim3(256), 0,
-                       st, g, b, p, mode);
+    hipLaunchKernelGGL((k_row_pow2<LR, TC, false, false>), dim3(nb),
+                       dim3(256), 0, st, g, b, p, mode);
   *nblocks = (int)nb * 4;  // partial sums per wavefront
   return hipGetLastError();
 }
