@@ -223,10 +223,10 @@ def adjoint_leg(reps=3):
     gl = ctypes.c_float()
     stream = torch.cuda.current_stream().cuda_stream
 
-    def fwd():
+    def fwd(ckpt=True):
         rc = L.scc_unrolled_forward(D.data_ptr(), M, K, S.data_ptr(), B, R, C,
                                     0.03, 50.0, 1.8, NIT, CK, Y.data_ptr(),
-                                    ck.data_ptr(), stream)
+                                    ck.data_ptr() if ckpt else None, stream)
         if rc:
             raise RuntimeError(L.scc_unrolled_last_error().decode())
 
@@ -239,27 +239,36 @@ def adjoint_leg(reps=3):
         if rc:
             raise RuntimeError(L.scc_unrolled_last_error().decode())
     print('bench: adjoint leg', file=sys.stderr, flush=True)
+    fwd(False)
     fwd()
     bwd()
     torch.cuda.synchronize()
-    tf, tfb = [], []
+    tp, tf, tfb = [], [], []
     for _ in range(reps):
         t0 = time.perf_counter()
-        fwd()
+        fwd(False)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        bwd()
+        fwd()
         torch.cuda.synchronize()
         t2 = time.perf_counter()
-        tf.append(t1 - t0)
-        tfb.append(t2 - t0)
-    f, fb = min(tf), min(tfb)
+        bwd()
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        tp.append(t1 - t0)
+        tf.append(t2 - t1)
+        tfb.append(t3 - t1)
+    p, f, fb = min(tp), min(tf), min(tfb)
     return {
         'workload': '%d images %dx%d, %d filters %dx%d, lambda 0.03, rho 50, '
                     '%d iterations, checkpoint every %d' % (B, R, C, M, K, K,
                                                              NIT, CK),
-        'forward_ms': 1000 * f, 'forward_adjoint_ms': 1000 * fb,
-        'ratio': fb / f,
+        'forward_plain_ms': 1000 * p,
+        'forward_ckpt_ms': 1000 * f, 'forward_adjoint_ms': 1000 * fb,
+        'ratio': fb / p,
+        'ratio_vs_checkpointed_forward': fb / f,
+        'note': 'ratio = (checkpointed forward + adjoint) / forward without '
+                'checkpoints or tape',
     }
 
 
@@ -283,6 +292,13 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-batched', action='store_true')
     ap.add_argument('--no-adjoint', action='store_true')
+    ap.add_argument('--no-plain', action='store_true',
+                    help='skip the plain-solver timing (secondary legs only)')
+    ap.add_argument('--batched-iters', type=int, default=150)
+    ap.add_argument('--copy-calibration', action='store_true',
+                    help='only run 4 device copies of 256 MiB (the reference '
+                         'kernel that scales the FETCH_SIZE / WRITE_SIZE '
+                         'counters to bytes) and exit')
     args = ap.parse_args()
     if args.gpus != 1:
         sys.stderr.write('bench.py measures one GPU: this build runs the '
@@ -293,6 +309,22 @@ def main():
     import numpy as np
     import torch
     from sporco_cuda import _abi
+
+    if args.copy_calibration:
+        a = torch.ones(64 * 1024 * 1024, device='cuda')
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(4):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({'copy_bytes': a.numel() * 4, 'copies': 4}))
+        return
+    if args.no_plain:
+        out = {'batched': None if args.no_batched else
+               batched_leg(iters=args.batched_iters),
+               'adjoint': None if args.no_adjoint else adjoint_leg()}
+        print(json.dumps(out), flush=True)
+        return
 
     wl = WORKLOADS[args.workload]
     R, C, M, K = wl['H'], wl['W'], wl['M'], wl['K']
@@ -371,7 +403,8 @@ def main():
         'kernels': per_kernel,
         'setup_ms': info['setup_ms'],
     }
-    out['batched'] = None if args.no_batched else batched_leg()
+    out['batched'] = None if args.no_batched else batched_leg(
+        iters=args.batched_iters)
     out['adjoint'] = None if args.no_adjoint else adjoint_leg()
     if not args.no_cpu:
         print('bench: cpu baseline (%.0f s budget)' % args.cpu_seconds,

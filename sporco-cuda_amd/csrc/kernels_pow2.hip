@@ -177,6 +177,9 @@ __global__ __launch_bounds__(256) void k_solve_pow2(Geom g, Buffers b,
   const int M = g.M;
   // Images of a batch (same size, shared dictionary): blockIdx.y.
   float2* const Fi = b.F + (long long)blockIdx.y * M * g.Hc;
+  // Optional copy of X-hat (the unrolled solver's tape), same layout.
+  float2* const Xt =
+      b.xtape ? b.xtape + (long long)blockIdx.y * M * g.Hc : nullptr;
   float2 sh = make_float2(0.f, 0.f);
   float gh = 0.f;
   if (valid) {
@@ -220,8 +223,9 @@ __global__ __launch_bounds__(256) void k_solve_pow2(Geom g, Buffers b,
     const int m = gi * MPT + i;
     if (valid && m < M) {
       const float inv = GRAD ? id[i] : irho;
-      Fi[(long long)m * g.Hc + f] =
-          cscale(csub(bb[i], cmul(cconj(dd[i]), cb)), inv);
+      const float2 x = cscale(csub(bb[i], cmul(cconj(dd[i]), cb)), inv);
+      Fi[(long long)m * g.Hc + f] = x;
+      if (Xt != nullptr) Xt[(long long)m * g.Hc + f] = x;
     }
   }
 }
@@ -237,11 +241,11 @@ __global__ __launch_bounds__(256) void k_solve_pow2(Geom g, Buffers b,
 // order of the real row as float2 pairs, so the update and the forward
 // transform of Y - U need no LDS pass at all.
 // --------------------------------------------------------------------------
-// TAPE (plain update only): also store V of every coefficient to b.vout,
-// the unrolled solver's tape.
-template <int LR, int TC, bool GENERAL, bool TAPE>
-__global__ __launch_bounds__(256) __attribute__((
-    amdgpu_waves_per_eu(GENERAL ? 1 : RowCfg<LR>::WAVES))) void
+// TAPE (plain update only) records the unrolled solver's tape: 1 the
+// support bits of V (|V| > t, sign V) to b.tbits, 2 V itself to b.vout.
+template <int LR, int TC, bool GENERAL, int TAPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
+    GENERAL ? 1 : (TAPE == 1 ? 3 : RowCfg<LR>::WAVES)))) void
 k_row_pow2(Geom g, Buffers b, Params p, int mode) {
   using RC = RowCfg<LR>;
   constexpr int E = RC::E, T = RC::T, RPW = RC::RPW;
@@ -267,6 +271,14 @@ k_row_pow2(Geom g, Buffers b, Params p, int mode) {
   const long long rowoff = (long long)row * C;
   float2* Y2 = reinterpret_cast<float2*>(b.Y + (long long)row0 * C);
   float2* U2 = reinterpret_cast<float2*>(b.U + (long long)row0 * C);
+  // Y and U are read from Yin / Uin when given (a checkpoint), written to
+  // Y / U.
+  const float2* Y2i = b.Yin ? reinterpret_cast<const float2*>(
+                                  b.Yin + (long long)row0 * C)
+                            : Y2;
+  const float2* U2i = b.Uin ? reinterpret_cast<const float2*>(
+                                  b.Uin + (long long)row0 * C)
+                            : U2;
   const int lo = rw * LR + t;  // this lane's first float2 of its row
 
   // Half spectrum in slot order, column L on lane 0; Y and U prefetched.
@@ -278,8 +290,8 @@ k_row_pow2(Geom g, Buffers b, Params p, int mode) {
   if (RC::PREFETCH) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      y2[e] = Y2[lo + T * e];
-      u2[e] = U2[lo + T * e];
+      y2[e] = Y2i[lo + T * e];
+      u2[e] = U2i[lo + T * e];
     }
   }
   // Inverse real-FFT pre-processing:
@@ -314,8 +326,8 @@ k_row_pow2(Geom g, Buffers b, Params p, int mode) {
   if (!RC::PREFETCH) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      y2[e] = Y2[lo + T * e];
-      u2[e] = U2[lo + T * e];
+      y2[e] = Y2i[lo + T * e];
+      u2[e] = U2i[lo + T * e];
     }
   }
   if (GENERAL) {
@@ -351,15 +363,25 @@ k_row_pow2(Geom g, Buffers b, Params p, int mode) {
     // Y = copysign(max(|V| - t, 0), V), U = V - Y.
     const float th = lr * wmap;
     const float a = p.alpha, a1 = 1.f - p.alpha;
+    constexpr int TW = TapeWords<E>::W;
+    unsigned tb[TW];
+#pragma unroll
+    for (int w = 0; w < TW; ++w) tb[w] = 0u;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const float x0 = z[e].x * invN, x1 = z[e].y * invN;
       const float v0 = fmaf(a, x0, fmaf(a1, y2[e].x, u2[e].x));
       const float v1 = fmaf(a, x1, fmaf(a1, y2[e].y, u2[e].y));
-      const float yn0 = copysignf(fmaxf(fabsf(v0) - th, 0.f), v0);
-      const float yn1 = copysignf(fmaxf(fabsf(v1) - th, 0.f), v1);
+      const float q0 = fabsf(v0) - th, q1 = fabsf(v1) - th;
+      const float yn0 = copysignf(fmaxf(q0, 0.f), v0);
+      const float yn1 = copysignf(fmaxf(q1, 0.f), v1);
       const float un0 = v0 - yn0, un1 = v1 - yn1;
-      if (TAPE)
+      if (TAPE == 1)
+        tb[e / 8] |= ((unsigned)(q0 > 0.f) | ((unsigned)(v0 < 0.f) << 1) |
+                      ((unsigned)(q1 > 0.f) << 2) |
+                      ((unsigned)(v1 < 0.f) << 3))
+                     << (4 * (e % 8));
+      if (TAPE == 2)
         reinterpret_cast<float2*>(b.vout + (long long)row0 * C)[lo + T * e] =
             make_float2(v0, v1);
       acc.x = fmaf(x0, x0, fmaf(x1, x1, acc.x));
@@ -372,6 +394,11 @@ k_row_pow2(Geom g, Buffers b, Params p, int mode) {
       Y2[lo + T * e] = make_float2(yn0, yn1);
       U2[lo + T * e] = make_float2(un0, un1);
       z[e] = make_float2(yn0 - un0, yn1 - un1);
+    }
+    if (TAPE == 1) {
+      unsigned* tw = b.tbits + ((long long)row * T + t) * TW;
+#pragma unroll
+      for (int w = 0; w < TW; ++w) tw[w] = tb[w];
     }
   }
   // Per-wavefront partial sums (no block barrier).
@@ -488,14 +515,17 @@ hipError_t row_launch(const Geom& g, const Buffers& b, const Params& p,
   const unsigned nb = (unsigned)((long long)g.M * g.R / RowCfg<LR>::RPB);
   const bool general = p.m0 || p.l1_mode == 2 || p.nonneg || p.nobndry;
   if (general)
-    hipLaunchKernelGGL((k_row_pow2<LR, TC, true, false>), dim3(nb), dim3(256),
-                       0, st, g, b, p, mode);
+    hipLaunchKernelGGL((k_row_pow2<LR, TC, true, 0>), dim3(nb), dim3(256), 0,
+                       st, g, b, p, mode);
+  else if (b.tbits != nullptr)
+    hipLaunchKernelGGL((k_row_pow2<LR, TC, false, 1>), dim3(nb), dim3(256), 0,
+                       st, g, b, p, mode);
   else if (b.vout != nullptr)
-    hipLaunchKernelGGL((k_row_pow2<LR, TC, false, true>), dim3(nb),
-                       dim3(256), 0, st, g, b, p, mode);
+    hipLaunchKernelGGL((k_row_pow2<LR, TC, false, 2>), dim3(nb), dim3(256), 0,
+                       st, g, b, p, mode);
   else
-    hipLaunchKernelGGL((k_row_pow2<LR, TC, false, false>), dim3(nb),
-                       dim3(256), 0, st, g, b, p, mode);
+    hipLaunchKernelGGL((k_row_pow2<LR, TC, false, 0>), dim3(nb), dim3(256), 0,
+                       st, g, b, p, mode);
   *nblocks = (int)nb * 4;  // partial sums per wavefront
   return hipGetLastError();
 }

@@ -27,6 +27,22 @@ struct RowCfg {
   static constexpr int WAVES = E <= 8 ? 4 : 2;
 };
 
+// Soft-threshold support tape of the unrolled solver: 4 bits per lane slot
+// e (one float pair v0, v1): bit 0 |v0| > t, bit 1 v0 < 0, bit 2 |v1| > t,
+// bit 3 v1 < 0.  Lane t of row `row` keeps TapeWords<E>::W words at
+// bits[(row * T + t) * W + w]; slot e is in word e / 8 at bit 4 (e % 8).
+template <int E>
+struct TapeWords {
+  static constexpr int W = (4 * E + 31) / 32;
+};
+
+// 32-bit words of the support tape per row of C columns (host side).
+inline int tape_words_per_row(int C) {
+  const int LR = C / 2;
+  const int E = LR <= 512 ? 8 : 16;
+  return (LR / E) * ((4 * E + 31) / 32);
+}
+
 __device__ __forceinline__ float2 shfl2(float2 v, int src) {
   return make_float2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
 }

@@ -105,6 +105,10 @@ struct Buffers {
   float2* twR;      // W_R^t, t < R
   float2* twC;      // W_C^t, t < C
   float* vout;      // optional copy of V = Xr + U (unrolled solve tape)
+  unsigned* tbits;  // optional support bits of V (unrolled solve tape)
+  float2* xtape;    // optional copy of X-hat after the solve (same layout)
+  const float* Yin; // optional source of the row kernel's Y (else Y)
+  const float* Uin; // optional source of the row kernel's U (else U)
 };
 
 // Kernel launchers (kernels.hip).

@@ -4,9 +4,13 @@
 // Forward: the solver's own kernels over B images of one size sharing the
 // dictionary (column and row kernels run over B*M maps, the solve takes the
 // image index), fixed rho, no stopping test; (Y, U) is saved every k
-// iterations.  Backward, per checkpoint segment from last to first: the
-// segment is recomputed from its checkpoint with a tape (X-hat after each
-// solve, V = Xr + U from each row kernel), then every iteration is reversed:
+// iterations, and the tape of the last segment is kept with the
+// checkpoints.  The tape of an iteration is X-hat after the solve (written
+// by the solve kernel) and the soft-threshold support of V = Xr + U as 4
+// bits per coefficient pair (|V| > t and sign V, written by the row
+// kernel).  Backward, per checkpoint segment from last to first: the
+// segment is recomputed from its checkpoint with a tape (not the last one),
+// then every iteration is reversed:
 //   k_adj_row1   shrink backward (support mask), relaxation backward, and the
 //                forward row transform of Xbar = alpha Vbar
 //   k_col_pow2   forward column transform            (adjoint of the inverse FFT)
@@ -14,6 +18,9 @@
 //                dictionary- and image-spectrum gradients, rho Bbar / w
 //   k_col_pow2   inverse column transform            (adjoint of the forward FFT)
 //   k_adj_row2   inverse row transform, Ybar += Abar, Ubar -= Abar
+// Inside a segment k_adj_row21 fuses k_adj_row2 of one iteration with
+// k_adj_row1 of the one before it, so Ybar and Ubar are read and written
+// once per iteration.
 // The FFT adjoints reuse the forward kernels; the half-spectrum column
 // multiplicity w (1 for DC / Nyquist, 2 otherwise) and 1/N are applied in
 // k_adj_solve.  Lambda's gradient is a deterministic FP64 sum of
@@ -39,18 +46,43 @@ namespace scc {
 // Adjoint kernels
 // --------------------------------------------------------------------------
 
-// Shrink / dual / relaxation backward on one row, then the forward real FFT
-// of Xbar into the row spectra of F.  Vbar = Ubar + (Ybar - Ubar) 1[|V| > t];
-// tbar += (Ubar - Ybar) sign(V) 1[|V| > t]; Xbar = a Vbar;
+// Shrink / dual / relaxation backward of one lane's slots, in place on
+// (yb, ub), from its support bits: Vbar = Ubar + (Ybar - Ubar) 1[|V| > t];
+// tbar += (Ubar - Ybar) sign(V) 1[|V| > t]; Xbar = a Vbar (returned in z);
 // Ybar <- (1 - a) Vbar; Ubar <- Vbar.
+template <int E>
+__device__ __forceinline__ void shrink_relax_bwd(
+    float2 (&yb)[E], float2 (&ub)[E], float2 (&z)[E],
+    const unsigned (&tb)[TapeWords<E>::W], float alpha, float& lacc) {
+  const float a1 = 1.f - alpha;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const unsigned q = (tb[e / 8] >> (4 * (e % 8))) & 15u;
+    const bool m0 = q & 1u, m1 = q & 4u;
+    const float vb0 = m0 ? yb[e].x : ub[e].x;
+    const float vb1 = m1 ? yb[e].y : ub[e].y;
+    if (m0) lacc += (q & 2u) ? yb[e].x - ub[e].x : ub[e].x - yb[e].x;
+    if (m1) lacc += (q & 8u) ? yb[e].y - ub[e].y : ub[e].y - yb[e].y;
+    yb[e] = make_float2(a1 * vb0, a1 * vb1);
+    ub[e] = make_float2(vb0, vb1);
+    z[e] = make_float2(alpha * vb0, alpha * vb1);
+  }
+}
+
+// Shrink / dual / relaxation backward on one row (support bits of the
+// iteration in tbits; Ybar read from Ybin when given, else Yb), then the
+// forward real FFT of Xbar into the row spectra of F; per-wavefront
+// partial sums of tbar in lpart.
 template <int LR, int TC>
-__global__ __launch_bounds__(256) void k_adj_row1(Geom g, float* Yb, float* Ub,
-                                                  const float* V, float thr,
+__global__ __launch_bounds__(256) void k_adj_row1(Geom g, const float* Ybin,
+                                                  float* Yb, float* Ub,
+                                                  const unsigned* tbits,
                                                   float alpha, float2* F,
                                                   const float2* twC,
                                                   float* lpart) {
   using RC = RowCfg<LR>;
   constexpr int E = RC::E, T = RC::T, RPW = RC::RPW;
+  constexpr int TW = TapeWords<E>::W;
   constexpr int C = 2 * LR;
   constexpr int LP = reg::lds_len(LR);
   __shared__ float2 lds[4 * RPW * LP];
@@ -68,29 +100,96 @@ __global__ __launch_bounds__(256) void k_adj_row1(Geom g, float* Yb, float* Ub,
   const long long base = (long long)row0 * C;
   float2* Y2 = reinterpret_cast<float2*>(Yb + base);
   float2* U2 = reinterpret_cast<float2*>(Ub + base);
-  const float2* V2 = reinterpret_cast<const float2*>(V + base);
   const int lo = rw * LR + t;
-  const float a1 = 1.f - alpha;
-  float lacc = 0.f;
-  float2 z[E];
+  unsigned tb[TW];
+  const unsigned* tw = tbits + ((long long)(row0 + rw) * T + t) * TW;
+#pragma unroll
+  for (int w = 0; w < TW; ++w) tb[w] = tw[w];
+  // Ybar is read from Ybin when given (dL/dY of the last iterate).
+  const float2* Y2i =
+      Ybin ? reinterpret_cast<const float2*>(Ybin + base) : Y2;
+  float2 yb[E], ub[E], z[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const float2 v = V2[lo + T * e];
-    const float2 yb = Y2[lo + T * e];
-    const float2 ub = U2[lo + T * e];
-    const bool m0 = fabsf(v.x) > thr, m1 = fabsf(v.y) > thr;
-    const float vb0 = m0 ? yb.x : ub.x;
-    const float vb1 = m1 ? yb.y : ub.y;
-    if (m0) lacc += (ub.x - yb.x) * (v.x < 0.f ? -1.f : 1.f);
-    if (m1) lacc += (ub.y - yb.y) * (v.y < 0.f ? -1.f : 1.f);
-    Y2[lo + T * e] = make_float2(a1 * vb0, a1 * vb1);
-    U2[lo + T * e] = make_float2(vb0, vb1);
-    z[e] = make_float2(alpha * vb0, alpha * vb1);
+    yb[e] = Y2i[lo + T * e];
+    ub[e] = U2[lo + T * e];
+  }
+  float lacc = 0.f;
+  shrink_relax_bwd<E>(yb, ub, z, tb, alpha, lacc);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    Y2[lo + T * e] = yb[e];
+    U2[lo + T * e] = ub[e];
   }
   const float ls = wave_sum(lacc);
   if (lane == 0) lpart[wave] = ls;
   reg::fft_wave<LR, E, 1>(z, t, L, twC, 2);
   store_spectrum<LR, TC>(z, t, partner, r, R, F + (long long)m * g.Hc, twC);
+}
+
+// k_adj_row2 of one iteration fused with k_adj_row1 of the previous one:
+// Abar = inverse row FFT of F; (Ybar, Ubar) += (Abar, -Abar) in registers;
+// then the shrink / relaxation backward with the previous iteration's
+// support bits and the forward row FFT of Xbar back into F (each row's
+// spectrum is read and written by the same wavefront).
+template <int LR, int TC>
+__global__ __launch_bounds__(256) void k_adj_row21(Geom g, float2* F,
+                                                   float* Yb, float* Ub,
+                                                   const unsigned* tbits,
+                                                   float alpha,
+                                                   const float2* twC,
+                                                   float* lpart) {
+  using RC = RowCfg<LR>;
+  constexpr int E = RC::E, T = RC::T, RPW = RC::RPW;
+  constexpr int TW = TapeWords<E>::W;
+  constexpr int C = 2 * LR;
+  constexpr int LP = reg::lds_len(LR);
+  __shared__ float2 lds[4 * RPW * LP];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int rw = lane / T;
+  const int t = lane & (T - 1);
+  const int partner = rw * T + ((T - t) & (T - 1));
+  const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wv);
+  const int row0 = wave * RPW;
+  const int R = g.R;
+  const int m = row0 >> g.lgR;
+  const int r = (row0 & (R - 1)) + rw;
+  float2* L = lds + (wv * RPW + rw) * LP;
+  float2* Fm = F + (long long)m * g.Hc;
+  const long long base = (long long)row0 * C;
+  float2* Y2 = reinterpret_cast<float2*>(Yb + base);
+  float2* U2 = reinterpret_cast<float2*>(Ub + base);
+  const int lo = rw * LR + t;
+  float2 z[E], yb[E], ub[E];
+  load_spectrum_c2r<LR, TC>(z, t, partner, r, R, Fm, twC);
+  unsigned tb[TW];
+  const unsigned* tw = tbits + ((long long)(row0 + rw) * T + t) * TW;
+#pragma unroll
+  for (int w = 0; w < TW; ++w) tb[w] = tw[w];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    yb[e] = Y2[lo + T * e];
+    ub[e] = U2[lo + T * e];
+  }
+  reg::fft_wave<LR, E, -1>(z, t, L, twC, 2);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    yb[e] = cadd(yb[e], z[e]);
+    ub[e] = csub(ub[e], z[e]);
+  }
+  float lacc = 0.f;
+  shrink_relax_bwd<E>(yb, ub, z, tb, alpha, lacc);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    Y2[lo + T * e] = yb[e];
+    U2[lo + T * e] = ub[e];
+  }
+  const float ls = wave_sum(lacc);
+  if (lane == 0) lpart[wave] = ls;
+  reg::wave_sync();
+  reg::fft_wave<LR, E, 1>(z, t, L, twC, 2);
+  store_spectrum<LR, TC>(z, t, partner, r, R, Fm, twC);
 }
 
 // Adjoint X-solve.  Per frequency, with D the dictionary spectra, Xh the
@@ -100,8 +199,15 @@ __global__ __launch_bounds__(256) void k_adj_row1(Geom g, float* Yb, float* Ub,
 //             + conj(Bbar_m) S
 //   Sbar   += sum_j D_j Bbar_j
 //   F_m     = rho Bbar_m / w
-// G lanes per frequency, MPT maps per lane; the images of the batch are
-// looped inside so that Dbar is read and written once per launch.
+// G lanes per frequency, MPT maps per lane.  A thread handles one
+// frequency for a group of up to kImgGroup images (blockIdx.y), looped
+// inside with Dbar accumulated in registers and added to that group's own
+// accumulator Dhb + group M Hc (summed over groups, in order, by
+// k_sum_groups after the last iteration).  Every load of an image step (F,
+// X-hat, S, the running Sbar) is issued before its arithmetic, so a step
+// waits for memory once.
+constexpr int kImgGroup = 16;
+
 template <int G, int MPT>
 __global__ __launch_bounds__(256) void k_adj_solve(
     Geom g, int nimg, float2* F, const float2* __restrict__ Xh,
@@ -135,11 +241,17 @@ __global__ __launch_bounds__(256) void k_adj_solve(
 #pragma unroll
   for (int o = FPW; o < 64; o <<= 1) sig += __shfl_xor(sig, o, 64);
   const float isig = 1.f / (rho + sig);
-  for (int img = 0; img < nimg; ++img) {
+  const int i0 = blockIdx.y * kImgGroup;
+  const int i1 = min(nimg, i0 + kImgGroup);
+  for (int img = i0; img < i1; ++img) {
     float2* Fi = F + (long long)img * M * g.Hc;
     const float2* Xi = Xh + (long long)img * M * g.Hc;
     float2 xb[MPT], xh[MPT];
-    float2 num = make_float2(0.f, 0.f);
+    float2 sh = make_float2(0.f, 0.f), sb = make_float2(0.f, 0.f);
+    if (valid) {
+      sh = Sf[(long long)img * g.Hc + f];
+      if (gi == 0) sb = Shb[(long long)img * g.Hc + f];
+    }
 #pragma unroll
     for (int i = 0; i < MPT; ++i) {
       const int m = gi * MPT + i;
@@ -147,10 +259,15 @@ __global__ __launch_bounds__(256) void k_adj_solve(
       xh[i] = make_float2(0.f, 0.f);
       if (valid && m < M) {
         const long long o = (long long)m * g.Hc + f;
-        xb[i] = cscale(Fi[o], sin_);
+        xb[i] = Fi[o];
         xh[i] = Xi[o];
-        num = cadd(num, cmul(dd[i], xb[i]));
       }
+    }
+    float2 num = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < MPT; ++i) {
+      xb[i] = cscale(xb[i], sin_);
+      num = cadd(num, cmul(dd[i], xb[i]));
     }
 #pragma unroll
     for (int o = FPW; o < 64; o <<= 1) {
@@ -172,8 +289,6 @@ __global__ __launch_bounds__(256) void k_adj_solve(
       sDX.x += __shfl_xor(sDX.x, o, 64);
       sDX.y += __shfl_xor(sDX.y, o, 64);
     }
-    const float2 sh =
-        valid ? Sf[(long long)img * g.Hc + f] : make_float2(0.f, 0.f);
 #pragma unroll
     for (int i = 0; i < MPT; ++i) {
       const int m = gi * MPT + i;
@@ -184,18 +299,26 @@ __global__ __launch_bounds__(256) void k_adj_solve(
         Fi[(long long)m * g.Hc + f] = cscale(xb[i], sout);
       }
     }
-    if (valid && gi == 0) {
-      float2* s = Shb + (long long)img * g.Hc + f;
-      *s = cadd(*s, sDB);
-    }
+    if (valid && gi == 0) Shb[(long long)img * g.Hc + f] = cadd(sb, sDB);
   }
+  float2* Dg = Dhb + (long long)blockIdx.y * M * g.Hc;
 #pragma unroll
   for (int i = 0; i < MPT; ++i) {
     const int m = gi * MPT + i;
     if (valid && m < M) {
-      float2* d = Dhb + (long long)m * g.Hc + f;
+      float2* d = Dg + (long long)m * g.Hc + f;
       *d = cadd(*d, dacc[i]);
     }
+  }
+}
+
+// Dhb[0] += Dhb[1] + ... + Dhb[ngrp - 1] (n complex values each, in order).
+__global__ void k_sum_groups(float2* Dhb, int ngrp, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    float2 a = Dhb[i];
+    for (int k = 1; k < ngrp; ++k) a = cadd(a, Dhb[(long long)k * n + i]);
+    Dhb[i] = a;
   }
 }
 
@@ -255,20 +378,36 @@ __global__ void k_crop(const float* src, float* dst, int M, int K, int R,
   dst[i] = src[((long long)m * R + a) * C + c];
 }
 
-// acc += sum of n partials (one block, fixed order, FP64).
-__global__ __launch_bounds__(1024) void k_sum_parts(const float* part, int n,
-                                                    double* acc) {
+// Sum of n partials in two fixed-order FP64 passes: 64 blocks each add a
+// contiguous chunk (k_lsum_blocks), one wavefront adds the 64 results
+// (k_lsum_final) into *acc.
+constexpr int kLsumBlocks = 64;
+
+__global__ __launch_bounds__(1024) void k_lsum_blocks(const float* part,
+                                                      long long n,
+                                                      double* bsum) {
   __shared__ double red[16];
+  const long long chunk = (n + gridDim.x - 1) / gridDim.x;
+  const long long i0 = blockIdx.x * chunk;
+  const long long i1 = min(n, i0 + chunk);
   double v = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) v += (double)part[i];
+  for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x)
+    v += (double)part[i];
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
     double s = 0.0;
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w];
-    *acc += s;
+    bsum[blockIdx.x] = s;
   }
+}
+
+__global__ __launch_bounds__(64) void k_lsum_final(const double* bsum,
+                                                   double* acc) {
+  double v = bsum[threadIdx.x];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if (threadIdx.x == 0) *acc = v;
 }
 
 // --------------------------------------------------------------------------
@@ -280,8 +419,8 @@ namespace {
   X(128, 16) X(128, 8) X(128, 4) X(256, 16) X(256, 8) X(256, 4) X(512, 16) \
       X(512, 8) X(512, 4) X(1024, 16) X(1024, 8) X(1024, 4)
 
-hipError_t adj_row1(const Geom& gbm, const Pow2Config& cfg, float* Yb,
-                    float* Ub, const float* V, float thr, float alpha,
+hipError_t adj_row1(const Geom& gbm, const Pow2Config& cfg, const float* Ybin,
+                    float* Yb, float* Ub, const unsigned* tbits, float alpha,
                     float2* F, const float2* twC, float* lpart, int* nparts,
                     hipStream_t st) {
   const int LR = gbm.C / 2;
@@ -290,7 +429,26 @@ hipError_t adj_row1(const Geom& gbm, const Pow2Config& cfg, float* Yb,
     const unsigned nb = (unsigned)((long long)gbm.M * gbm.R /              \
                                    RowCfg<lr>::RPB);                        \
     hipLaunchKernelGGL((k_adj_row1<lr, tc>), dim3(nb), dim3(256), 0, st,   \
-                       gbm, Yb, Ub, V, thr, alpha, F, twC, lpart);         \
+                       gbm, Ybin, Yb, Ub, tbits, alpha, F, twC, lpart);    \
+    *nparts = (int)nb * 4;                                                 \
+    return hipGetLastError();                                              \
+  }
+  SCC_UNR_CASES(X)
+#undef X
+  return hipErrorNotSupported;
+}
+
+hipError_t adj_row21(const Geom& gbm, const Pow2Config& cfg, float2* F,
+                     float* Yb, float* Ub, const unsigned* tbits, float alpha,
+                     const float2* twC, float* lpart, int* nparts,
+                     hipStream_t st) {
+  const int LR = gbm.C / 2;
+#define X(lr, tc)                                                          \
+  if (LR == lr && cfg.TC == tc) {                                          \
+    const unsigned nb = (unsigned)((long long)gbm.M * gbm.R /              \
+                                   RowCfg<lr>::RPB);                        \
+    hipLaunchKernelGGL((k_adj_row21<lr, tc>), dim3(nb), dim3(256), 0, st,  \
+                       gbm, F, Yb, Ub, tbits, alpha, twC, lpart);          \
     *nparts = (int)nb * 4;                                                 \
     return hipGetLastError();                                              \
   }
@@ -318,18 +476,21 @@ hipError_t adj_row2(const Geom& gbm, const Pow2Config& cfg, float2* F,
   return hipErrorNotSupported;
 }
 
+int img_groups(int nimg) { return (nimg + kImgGroup - 1) / kImgGroup; }
+
 hipError_t adj_solve(const Geom& g, int nimg, float2* F, const float2* Xh,
                      const float2* Df, const float2* Sf, float2* Dhb,
                      float2* Shb, float rho, hipStream_t st) {
   int G = 1;
   while (G * 8 < g.M) G *= 2;
   const long long fpb = 4LL * (64 / G);
-  const unsigned nb = (unsigned)((g.Hc + fpb - 1) / fpb);
+  const dim3 nb((unsigned)((g.Hc + fpb - 1) / fpb),
+                (unsigned)img_groups(nimg));
   const float invN = 1.f / (float)g.N;
 #define X(gg)                                                                \
   if (G == gg) {                                                             \
-    hipLaunchKernelGGL((k_adj_solve<gg, 8>), dim3(nb), dim3(256), 0, st, g,  \
-                       nimg, F, Xh, Df, Sf, Dhb, Shb, rho, invN);            \
+    hipLaunchKernelGGL((k_adj_solve<gg, 8>), nb, dim3(256), 0, st, g, nimg,  \
+                       F, Xh, Df, Sf, Dhb, Shb, rho, invN);                  \
     return hipGetLastError();                                                \
   }
   X(1) X(2) X(4) X(8) X(16) X(32)
@@ -385,7 +546,8 @@ struct UWs {
   Geom g{}, gbm{};
   Pow2Config cfg{};
   UBuf F, Df, Sf, Y, U, part, bsum, ctrl, twR, twC;
-  UBuf tXh, tV, Yb, Ub, Dhb, Shb, lacc, tmp;
+  UBuf tXh, tbits, Yb, Ub, Dhb, Shb, lacc, tmp, lparts;
+  size_t nr = 0, nc = 0, nbits = 0;  // per-iteration tape sizes
 };
 
 std::mutex u_mu;
@@ -439,6 +601,10 @@ int get_uws(int R, int C, int M, int B, int k, UWs** out) {
   w->gbm = g;
   w->gbm.M = B * M;
   const size_t nr = (size_t)B * M * g.N, nc = (size_t)B * M * g.Hc;
+  const size_t nbits = (size_t)B * M * R * tape_words_per_row(C);
+  w->nr = nr;
+  w->nc = nc;
+  w->nbits = nbits;
   const size_t nwaves = (size_t)pow2_row_blocks(w->gbm, w->cfg);
   int rc;
   if ((rc = w->F.get(nc * 8)) || (rc = w->Df.get((size_t)M * g.Hc * 8)) ||
@@ -447,8 +613,10 @@ int get_uws(int R, int C, int M, int B, int k, UWs** out) {
       (rc = w->bsum.get(64 * 5 * 8)) || (rc = w->ctrl.get(sizeof(Ctrl))) ||
       (rc = w->twR.get((size_t)R * 8)) || (rc = w->twC.get((size_t)C * 8)) ||
       (rc = w->tXh.get((size_t)k * nc * 8)) ||
-      (rc = w->tV.get((size_t)k * nr * 4)) || (rc = w->Yb.get(nr * 4)) ||
-      (rc = w->Ub.get(nr * 4)) || (rc = w->Dhb.get((size_t)M * g.Hc * 8)) ||
+      (rc = w->tbits.get((size_t)k * nbits * 4)) ||
+      (rc = w->Yb.get(nr * 4)) ||
+      (rc = w->Ub.get(nr * 4)) ||
+      (rc = w->Dhb.get((size_t)img_groups(B) * M * g.Hc * 8)) ||
       (rc = w->Shb.get((size_t)B * g.Hc * 8)) || (rc = w->lacc.get(8)) ||
       (rc = w->tmp.get((size_t)M * g.N * 4)))
     return rc;
@@ -514,21 +682,62 @@ int setup(UWs* w, const float* D, int K, const float* S, float rho,
 }
 
 // One forward iteration over the batch; F holds the row spectra of Y - U.
-// With a tape slot, X-hat after the solve and V are recorded.
-int fwd_iter(UWs* w, const Params& p, float2* tXh, float* tV,
-             hipStream_t st) {
+// With tape pointers the solve also writes X-hat to xtape and the row
+// kernel the support bits of V to tbits (or V itself to vout).  The row
+// kernel reads (Y, U) from (Yin, Uin) when given and writes them to
+// (Yout, Uout) when given, else in place in the workspace.
+struct IterIO {
+  const float* Yin = nullptr;
+  const float* Uin = nullptr;
+  float* Yout = nullptr;
+  float* Uout = nullptr;
+};
+
+int fwd_iter(UWs* w, const Params& p, float2* xtape, unsigned* tbits,
+             float* vout, const IterIO& io, hipStream_t st) {
   Buffers b = make_buffers(w);
-  b.vout = tV;
+  b.xtape = xtape;
+  b.tbits = tbits;
+  b.vout = vout;
   const int BM = w->B * w->M;
   int nparts = 0;
   UHIP(pow2_col(w->g, w->cfg, BM, b.F, b.twR, 0, b.ctrl, st));
   UHIP(pow2_solve(w->g, w->cfg, b, p, st, w->B));
-  if (tXh != nullptr)
-    UHIP(hipMemcpyAsync(tXh, b.F, (size_t)BM * w->g.Hc * 8,
-                        hipMemcpyDeviceToDevice, st));
+  b.xtape = nullptr;
   UHIP(pow2_col(w->g, w->cfg, BM, b.F, b.twR, 1, b.ctrl, st));
+  // Inputs default to the workspace, also when the outputs are redirected.
+  b.Yin = io.Yin ? io.Yin : b.Y;
+  b.Uin = io.Uin ? io.Uin : b.U;
+  if (io.Yout) b.Y = io.Yout;
+  if (io.Uout) b.U = io.Uout;
   UHIP(pow2_row(w->gbm, w->cfg, b, p, kRowFused, &nparts, st));
   return SCC_OK;
+}
+
+// Checkpoint buffer: (Y, U) at the start of every segment, then the tape
+// of the last segment (X-hat of each iteration, then its support bits).
+struct CkptLayout {
+  int nseg, it0_last, cnt_last;
+  size_t tail;  // float offset of the last segment's tape
+};
+
+CkptLayout ckpt_layout(int n_iter, int k, size_t nr) {
+  CkptLayout c;
+  c.nseg = (n_iter + k - 1) / k;
+  c.it0_last = (c.nseg - 1) * k;
+  c.cnt_last = n_iter - c.it0_last;
+  c.tail = (size_t)c.nseg * 2 * nr;
+  return c;
+}
+
+float2* tail_xh(const UWs* w, float* ckpt, const CkptLayout& c, int j) {
+  return reinterpret_cast<float2*>(ckpt + c.tail) + (size_t)j * w->nc;
+}
+
+unsigned* tail_bits(const UWs* w, float* ckpt, const CkptLayout& c, int j) {
+  return reinterpret_cast<unsigned*>(ckpt + c.tail +
+                                     (size_t)c.cnt_last * 2 * w->nc) +
+         (size_t)j * w->nbits;
 }
 
 }  // namespace
@@ -540,10 +749,13 @@ extern "C" {
 
 size_t scc_unrolled_ckpt_floats(int M, int B, int R, int C, int n_iter,
                                 int ckpt_every) {
-  if (M < 1 || B < 1 || R < 1 || C < 1 || n_iter < 1 || ckpt_every < 1)
+  if (M < 1 || B < 1 || R < 2 || C < 2 || n_iter < 1 || ckpt_every < 1)
     return 0;
-  const size_t nseg = (size_t)(n_iter + ckpt_every - 1) / ckpt_every;
-  return nseg * 2 * (size_t)B * M * R * C;
+  const size_t nr = (size_t)B * M * R * C;
+  const size_t nc = (size_t)B * M * R * (C / 2 + 1);
+  const size_t nbits = (size_t)B * M * R * tape_words_per_row(C);
+  const CkptLayout c = ckpt_layout(n_iter, ckpt_every, nr);
+  return c.tail + (size_t)c.cnt_last * (2 * nc + nbits);
 }
 
 int scc_unrolled_forward(const float* D, int M, int K, const float* S, int B,
@@ -552,27 +764,42 @@ int scc_unrolled_forward(const float* D, int M, int K, const float* S, int B,
                          void* stream) {
   int rc = check_shapes(M, K, B, R, C, n_iter, ckpt_every, rho);
   if (rc) return rc;
-  if (!D || !S || !Y || !ckpt) return ufail(SCC_ERR_INVALID, "NULL pointer");
+  if (!D || !S || !Y) return ufail(SCC_ERR_INVALID, "NULL pointer");
   std::lock_guard<std::mutex> lk(u_mu);
   UWs* w;
   if ((rc = get_uws(R, C, M, B, ckpt_every, &w))) return rc;
   hipStream_t st = (hipStream_t)stream;
   if ((rc = setup(w, D, K, S, rho, st))) return rc;
-  const size_t nr = (size_t)B * M * w->g.N;
-  UHIP(hipMemsetAsync(w->Y.p, 0, nr * 4, st));
-  UHIP(hipMemsetAsync(w->U.p, 0, nr * 4, st));
-  UHIP(hipMemsetAsync(w->F.p, 0, (size_t)B * M * w->g.Hc * 8, st));
+  const size_t nr = w->nr;
   const Params p = make_params(lambda, alpha);
+  const CkptLayout c = ckpt_layout(n_iter, ckpt_every, nr);
+  // Iteration i reads (Y, U) from its segment's checkpoint slot when it
+  // starts a segment and writes them into the next slot when it ends one,
+  // so no copies are made; the last iteration writes Y to the output.
+  auto slot = [&](int sg) { return ckpt + (size_t)sg * 2 * nr; };
+  float* y0 = ckpt ? slot(0) : (float*)w->Y.p;
+  UHIP(hipMemsetAsync(y0, 0, nr * 4, st));
+  UHIP(hipMemsetAsync(ckpt ? y0 + nr : (float*)w->U.p, 0, nr * 4, st));
+  UHIP(hipMemsetAsync(w->F.p, 0, w->nc * 8, st));
   for (int i = 0; i < n_iter; ++i) {
-    if (i % ckpt_every == 0) {
-      float* c = ckpt + (size_t)(i / ckpt_every) * 2 * nr;
-      UHIP(hipMemcpyAsync(c, w->Y.p, nr * 4, hipMemcpyDeviceToDevice, st));
-      UHIP(hipMemcpyAsync(c + nr, w->U.p, nr * 4, hipMemcpyDeviceToDevice,
-                          st));
+    IterIO io;
+    const bool last = i == n_iter - 1;
+    if (ckpt && i % ckpt_every == 0) {
+      io.Yin = slot(i / ckpt_every);
+      io.Uin = io.Yin + nr;
     }
-    if ((rc = fwd_iter(w, p, nullptr, nullptr, st))) return rc;
+    if (ckpt && !last && (i + 1) % ckpt_every == 0) {
+      io.Yout = slot((i + 1) / ckpt_every);
+      io.Uout = io.Yout + nr;
+    }
+    if (last) io.Yout = Y;
+    const int j = i - c.it0_last;  // tape slot in the last segment
+    const bool tape = ckpt && j >= 0;
+    if ((rc = fwd_iter(w, p, tape ? tail_xh(w, ckpt, c, j) : nullptr,
+                       tape ? tail_bits(w, ckpt, c, j) : nullptr, nullptr,
+                       io, st)))
+      return rc;
   }
-  UHIP(hipMemcpyAsync(Y, w->Y.p, nr * 4, hipMemcpyDeviceToDevice, st));
   UHIP(hipStreamSynchronize(st));
   return SCC_OK;
 }
@@ -592,8 +819,7 @@ int scc_unrolled_backward(const float* D, int M, int K, const float* S, int B,
   hipStream_t st = (hipStream_t)stream;
   if ((rc = setup(w, D, K, S, rho, st))) return rc;
   const Geom& g = w->g;
-  const size_t nr = (size_t)B * M * g.N;
-  const size_t nc = (size_t)B * M * g.Hc;
+  const size_t nr = w->nr;
   const int BM = B * M;
   Buffers b = make_buffers(w);
   const Params p = make_params(lambda, alpha);
@@ -602,39 +828,69 @@ int scc_unrolled_backward(const float* D, int M, int K, const float* S, int B,
   float2* Dhb = (float2*)w->Dhb.p;
   float2* Shb = (float2*)w->Shb.p;
   double* lacc = (double*)w->lacc.p;
-  UHIP(hipMemcpyAsync(Yb, gY, nr * 4, hipMemcpyDeviceToDevice, st));
+  float* ck = const_cast<float*>(ckpt);  // the tape is only read here
+  const CkptLayout c = ckpt_layout(n_iter, ckpt_every, nr);
   UHIP(hipMemsetAsync(Ub, 0, nr * 4, st));
-  UHIP(hipMemsetAsync(Dhb, 0, (size_t)M * g.Hc * 8, st));
+  const int ngrp = img_groups(B);
+  UHIP(hipMemsetAsync(Dhb, 0, (size_t)ngrp * M * g.Hc * 8, st));
   UHIP(hipMemsetAsync(Shb, 0, (size_t)B * g.Hc * 8, st));
-  UHIP(hipMemsetAsync(lacc, 0, 8, st));
-  const float thr = lambda / rho;
-  const int nseg = (n_iter + ckpt_every - 1) / ckpt_every;
-  for (int s = nseg - 1; s >= 0; --s) {
+  // Per-wavefront partial sums of lambda-bar, one slice per iteration,
+  // added up once after the last iteration.
+  const size_t nparts = (size_t)pow2_row_blocks(w->gbm, w->cfg);
+  if ((rc = w->lparts.get((size_t)n_iter * nparts * 4))) return rc;
+  float* lparts = (float*)w->lparts.p;
+  int slot = 0;
+  int np = 0;
+  for (int s = c.nseg - 1; s >= 0; --s) {
     const int it0 = s * ckpt_every;
     const int cnt = std::min(n_iter, it0 + ckpt_every) - it0;
-    // Restore the segment's start state and its first row spectra.
-    const float* c = ckpt + (size_t)s * 2 * nr;
-    UHIP(hipMemcpyAsync(b.Y, c, nr * 4, hipMemcpyDeviceToDevice, st));
-    UHIP(hipMemcpyAsync(b.U, c + nr, nr * 4, hipMemcpyDeviceToDevice, st));
-    UHIP(pow2_row_fwd(w->gbm, w->cfg, BM, kFwdYU, nullptr, b.Y, b.U, b.F,
-                      b.twC, b.ctrl, st));
-    for (int j = 0; j < cnt; ++j)
-      if ((rc = fwd_iter(w, p, (float2*)w->tXh.p + (size_t)j * nc,
-                         (float*)w->tV.p + (size_t)j * nr, st)))
-        return rc;
-    for (int j = cnt - 1; j >= 0; --j) {
-      int nparts = 0;
-      UHIP(adj_row1(w->gbm, w->cfg, Yb, Ub, (float*)w->tV.p + (size_t)j * nr,
-                    thr, alpha, b.F, b.twC, b.partials, &nparts, st));
-      hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(1024), 0, st, b.partials,
-                         nparts, lacc);
-      UHIP(hipGetLastError());
-      UHIP(pow2_col(g, w->cfg, BM, b.F, b.twR, 0, nullptr, st));
-      UHIP(adj_solve(g, B, b.F, (float2*)w->tXh.p + (size_t)j * nc, b.Df,
-                     b.Sf, Dhb, Shb, rho, st));
-      UHIP(pow2_col(g, w->cfg, BM, b.F, b.twR, 1, nullptr, st));
-      UHIP(adj_row2(w->gbm, w->cfg, b.F, Yb, Ub, nullptr, b.twC, 0, st));
+    const bool last = s == c.nseg - 1;
+    auto xh = [&](int j) {
+      return last ? tail_xh(w, ck, c, j)
+                  : (float2*)w->tXh.p + (size_t)j * w->nc;
+    };
+    auto bits = [&](int j) {
+      return last ? tail_bits(w, ck, c, j)
+                  : (unsigned*)w->tbits.p + (size_t)j * w->nbits;
+    };
+    if (!last) {
+      // Recompute the segment from its checkpoint (read in place: rsf is 1,
+      // so the row transform of Y - U does not write U), recording the tape.
+      float* cy = ck + (size_t)s * 2 * nr;
+      UHIP(pow2_row_fwd(w->gbm, w->cfg, BM, kFwdYU, nullptr, cy, cy + nr,
+                        b.F, b.twC, b.ctrl, st));
+      for (int j = 0; j < cnt; ++j) {
+        IterIO io;
+        if (j == 0) {
+          io.Yin = cy;
+          io.Uin = cy + nr;
+        }
+        if ((rc = fwd_iter(w, p, xh(j), bits(j), nullptr, io, st))) return rc;
+      }
     }
+    // dL/dY enters through the first shrink backward.
+    UHIP(adj_row1(w->gbm, w->cfg, s == c.nseg - 1 ? gY : nullptr, Yb, Ub,
+                  bits(cnt - 1), alpha, b.F, b.twC,
+                  lparts + (size_t)slot++ * nparts, &np, st));
+    for (int j = cnt - 1; j >= 0; --j) {
+      UHIP(pow2_col(g, w->cfg, BM, b.F, b.twR, 0, nullptr, st));
+      UHIP(adj_solve(g, B, b.F, xh(j), b.Df, b.Sf, Dhb, Shb, rho, st));
+      UHIP(pow2_col(g, w->cfg, BM, b.F, b.twR, 1, nullptr, st));
+      if (j > 0) {
+        UHIP(adj_row21(w->gbm, w->cfg, b.F, Yb, Ub, bits(j - 1), alpha,
+                       b.twC, lparts + (size_t)slot++ * nparts, &np, st));
+      } else {
+        UHIP(adj_row2(w->gbm, w->cfg, b.F, Yb, Ub, nullptr, b.twC, 0, st));
+      }
+    }
+  }
+  if (ngrp > 1) {
+    const long long n = (long long)M * g.Hc;
+    hipLaunchKernelGGL(k_sum_groups,
+                       dim3((unsigned)std::min<long long>((n + 255) / 256,
+                                                          65536)),
+                       dim3(256), 0, st, Dhb, ngrp, n);
+    UHIP(hipGetLastError());
   }
   // Image gradient: N irfft2(Sbar / w); dictionary: crop(N irfft2(Dbar / w)).
   Geom gB = g;
@@ -649,6 +905,12 @@ int scc_unrolled_backward(const float* D, int M, int K, const float* S, int B,
   const int nk = M * K * K;
   hipLaunchKernelGGL(k_crop, dim3((nk + 255) / 256), dim3(256), 0, st,
                      (const float*)w->tmp.p, gD, M, K, R, C);
+  UHIP(hipGetLastError());
+  hipLaunchKernelGGL(k_lsum_blocks, dim3(kLsumBlocks), dim3(1024), 0, st,
+                     (const float*)lparts, (long long)slot * (long long)nparts,
+                     b.blocksum);
+  hipLaunchKernelGGL(k_lsum_final, dim3(1), dim3(64), 0, st,
+                     (const double*)b.blocksum, lacc);
   UHIP(hipGetLastError());
   double l = 0.0;
   UHIP(hipMemcpyAsync(&l, lacc, 8, hipMemcpyDeviceToHost, st));
@@ -668,13 +930,15 @@ int scc_unrolled_tape(const float* D, int M, int K, const float* S, int B,
   if ((rc = get_uws(R, C, M, B, 1, &w))) return rc;
   hipStream_t st = (hipStream_t)stream;
   if ((rc = setup(w, D, K, S, rho, st))) return rc;
-  const size_t nr = (size_t)B * M * w->g.N;
+  const size_t nr = w->nr;
   UHIP(hipMemsetAsync(w->Y.p, 0, nr * 4, st));
   UHIP(hipMemsetAsync(w->U.p, 0, nr * 4, st));
-  UHIP(hipMemsetAsync(w->F.p, 0, (size_t)B * M * w->g.Hc * 8, st));
+  UHIP(hipMemsetAsync(w->F.p, 0, w->nc * 8, st));
   const Params p = make_params(lambda, alpha);
   for (int i = 0; i < n_iter; ++i)
-    if ((rc = fwd_iter(w, p, nullptr, V + (size_t)i * nr, st))) return rc;
+    if ((rc = fwd_iter(w, p, nullptr, nullptr, V + (size_t)i * nr, IterIO(),
+                       st)))
+      return rc;
   UHIP(hipStreamSynchronize(st));
   return SCC_OK;
 }
