@@ -124,6 +124,18 @@ const char *scc_last_error(void);
 void scc_set_history(int enabled);
 int scc_last_history(float *rsr, int cap);
 
+/* Objective after every iteration of later solves on this thread (always
+ * when Verbose): rows of (Fnc, DFid, Regl1, Regl2grd) with
+ *   DFid = 1/2 || sum_m d_m * z_m - s ||^2 (1/2 || W (...) ||^2, impulse
+ *          map excluded, for the masked variants),
+ *   Regl1 = || w z ||_1,  Regl2grd = 1/2 sum_m w^g_m || grad z_m ||^2,
+ *   Fnc = DFid + lambda Regl1 (+ mu Regl2grd),
+ * z = X, or Y with AuxVarObj (sporco's definitions; the reference's
+ * objective, cbpdn.cu:438-517, is not followed).  scc_last_objective
+ * copies up to cap rows and returns the number of iterations. */
+void scc_set_objective(int enabled);
+int scc_last_objective(float *rows, int cap);
+
 /* Per-stage device timing of later solves on this thread (HIP events on
  * the solve stream around each kernel of every iteration).  Stages:
  * 0 forward column FFT, 1 X-solve, 2 inverse column FFT, 3 row kernel

@@ -102,6 +102,8 @@ namespace {
 thread_local scc_info g_info;
 thread_local bool g_hist_on = false;
 thread_local std::vector<float> g_hist;
+thread_local bool g_obj_on = false;
+thread_local std::vector<float> g_obj;  // (Fnc, DFid, Regl1, Regl2g) rows
 // Per-stage device timing of the ADMM iterations (events on the solve
 // stream around every kernel): index = Stage.
 enum Stage { kStColFwd = 0, kStSolve, kStColInv, kStRow, kStReduce, kStRowFwd,
@@ -156,6 +158,7 @@ struct Workspace {
   Pow2Config p2{};
   DevBuf Y, U, F, Df, Sf, ghg, partials, blocksum, ctrl, hist, twR, twC;
   DevBuf sD, sS, sL1, sGW, sMask;  // staging for the host-pointer entry
+  DevBuf oG, oZ, oE, oPart, oVal, oHist;  // objective evaluation
   bool ghg_ready = false;
   hipStream_t stream = nullptr;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -347,6 +350,79 @@ struct DeviceGuard {
   }
 };
 
+// 2-D transforms of nmaps maps with the workspace's kernels: forward real
+// transform into spectra F, inverse into out times 1/N (F is destroyed).
+hipError_t ws_fwd2(Workspace* ws, int nmaps, const float* src, float2* F,
+                   hipStream_t st) {
+  const Geom& g = ws->g;
+  const float2* twR = (const float2*)ws->twR.p;
+  const float2* twC = (const float2*)ws->twC.p;
+  hipError_t e;
+  if (ws->pow2) {
+    e = pow2_row_fwd(g, ws->p2, nmaps, kFwdReal, src, nullptr, nullptr, F,
+                     twC, nullptr, st);
+    if (e == hipSuccess) e = pow2_col(g, ws->p2, nmaps, F, twR, 0, nullptr, st);
+  } else {
+    e = launch_row_fwd(g, nmaps, ws->rowp, kFwdReal, src, nullptr, nullptr, F,
+                       twC, nullptr, st);
+    if (e == hipSuccess) e = launch_col(g, nmaps, ws->colp, F, twR, 0, nullptr, st);
+  }
+  return e;
+}
+
+hipError_t ws_inv2(Workspace* ws, int nmaps, float2* F, float* out,
+                   hipStream_t st) {
+  const Geom& g = ws->g;
+  const float2* twR = (const float2*)ws->twR.p;
+  const float2* twC = (const float2*)ws->twC.p;
+  const float s = 1.f / (float)g.N;
+  hipError_t e;
+  if (ws->pow2) {
+    e = pow2_col(g, ws->p2, nmaps, F, twR, 1, nullptr, st);
+    if (e == hipSuccess) e = pow2_row_inv(g, ws->p2, nmaps, F, out, twC, s, st);
+  } else {
+    e = launch_col(g, nmaps, ws->colp, F, twR, 1, nullptr, st);
+    if (e == hipSuccess)
+      e = launch_row_inv(g, nmaps, ws->rowp, F, out, twC, s, st);
+  }
+  return e;
+}
+
+// Objective of iteration iter (Fnc, DFid, Regl1, Regl2g) into ws->oVal and
+// row iter - 1 of ws->oHist.  With aux off the variable is X, whose
+// spectrum was copied to ws->oG right after the solve; with aux on it is Y.
+int objective_step(Workspace* ws, const Buffers& b, const Params& p, bool aux,
+                   int iter, hipStream_t st) {
+  const Geom& g = ws->g;
+  const int M = g.M;
+  const int nf = obj_blocks(g);
+  const int nl = 512, nm = p.m0 ? 256 : 0;
+  double* pf = (double*)ws->oPart.p;
+  double* pl = pf + 2 * nf;
+  double* pm = pl + nl;
+  float2* G = (float2*)ws->oG.p;
+  float* Z = (float*)ws->oZ.p;
+  float2* E = (float2*)ws->oE.p;
+  float* e = (float*)(E + g.Hc);
+  if (aux) {
+    SCC_HIP(launch_obj_l1(g, p, b.Y, b.l1w, b.ctrl, pl, nl, iter, st));
+    SCC_HIP(ws_fwd2(ws, M, b.Y, G, st));
+    SCC_HIP(launch_obj_freq(g, b, p, G, p.m0 ? E : nullptr, pf, iter, st));
+  } else {
+    SCC_HIP(launch_obj_freq(g, b, p, G, p.m0 ? E : nullptr, pf, iter, st));
+    SCC_HIP(ws_inv2(ws, M, G, Z, st));
+    SCC_HIP(launch_obj_l1(g, p, Z, b.l1w, b.ctrl, pl, nl, iter, st));
+  }
+  if (p.m0) {
+    SCC_HIP(ws_inv2(ws, 1, E, e, st));
+    SCC_HIP(launch_obj_mask(g, e, b.mask, b.ctrl, pm, nm, iter, st));
+  }
+  SCC_HIP(launch_obj_final(pf, nf, pl, nl, pm, nm, p, 1.0 / (double)g.N,
+                           b.ctrl, (float*)ws->oVal.p, (float*)ws->oHist.p,
+                           iter, st));
+  return SCC_OK;
+}
+
 // The ADMM solve on device-resident inputs.  D (M,Kh,Kw), S (R,C), weights
 // and mask are device pointers; Y (M,R,C) receives the result on the device.
 int solve_core(Workspace* ws, const Resolved& rs, const float* D,
@@ -390,6 +466,18 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
   if (grad && (rc = ensure_ghg(ws))) return rc;
   const int maxit = o.MaxMainIter;
   if (p.track && (rc = ensure(ws->hist, (size_t)3 * std::max(maxit, 1) * 4)))
+    return rc;
+  // Objective evaluation after every iteration (Verbose, or requested).
+  const bool obj_on = g_obj_on || o.Verbose == 1;
+  const bool aux = o.AuxVarObj == 1;
+  if (obj_on &&
+      ((rc = ensure(ws->oG, (size_t)M * g.Hc * 8)) ||
+       (rc = ensure(ws->oZ, aux ? 4 : (size_t)M * g.N * 4)) ||
+       (rc = ensure(ws->oE, rs.mask ? (size_t)g.Hc * 8 + (size_t)g.N * 4
+                                     : 8)) ||
+       (rc = ensure(ws->oPart, ((size_t)2 * obj_blocks(g) + 512 + 256) * 8)) ||
+       (rc = ensure(ws->oVal, 4 * 4)) ||
+       (rc = ensure(ws->oHist, (size_t)4 * std::max(maxit, 1) * 4))))
     return rc;
 
   Buffers b{};
@@ -491,6 +579,9 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
       SCC_HIP(pow2_solve(g, ws->p2, b, p, st, 1, pending ? i - 1 : 0,
                          pending ? pow2_defer_blocks(g, M) : 0));
       SCC_HIP(mark(kStSolve));
+      if (obj_on && !aux)  // spectrum of X for the objective
+        SCC_HIP(hipMemcpyAsync(ws->oG.p, b.F, (size_t)M * g.Hc * 8,
+                               hipMemcpyDeviceToDevice, st));
       SCC_HIP(pow2_col(g, ws->p2, M, b.F, b.twR, 1, b.ctrl, st));
       SCC_HIP(mark(kStColInv));
       SCC_HIP(pow2_row(g, ws->p2, b, p, rmode, &nblocks, st));
@@ -499,6 +590,9 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
       SCC_HIP(mark(kStColFwd));
       SCC_HIP(launch_solve(g, b, p, st));
       SCC_HIP(mark(kStSolve));
+      if (obj_on && !aux)  // spectrum of X for the objective
+        SCC_HIP(hipMemcpyAsync(ws->oG.p, b.F, (size_t)M * g.Hc * 8,
+                               hipMemcpyDeviceToDevice, st));
       SCC_HIP(launch_col(g, M, ws->colp, b.F, b.twR, 1, b.ctrl, st));
       SCC_HIP(mark(kStColInv));
       SCC_HIP(launch_row(g, ws->rowp, b, p, rmode, &nblocks, st));
@@ -518,18 +612,31 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
                                b.twC, b.ctrl, st));
       SCC_HIP(mark(kStRowFwd));
     }
+    if (obj_on && (rc = objective_step(ws, b, p, aux, i, st))) return rc;
     if ((can_stop || o.Verbose == 1) && (i % poll == 0 || i == maxit)) {
       SCC_HIP(hipMemcpyAsync(ws->hctrl, b.ctrl, sizeof(Ctrl),
                              hipMemcpyDeviceToHost, st));
+      float ov[4] = {0.f, 0.f, 0.f, 0.f};
+      if (o.Verbose == 1)
+        SCC_HIP(hipMemcpyAsync(ov, ws->oVal.p, sizeof(ov),
+                               hipMemcpyDeviceToHost, st));
       SCC_HIP(hipStreamSynchronize(st));
       if (o.Verbose == 1) {
         if (i == 1) {
-          std::printf("Itn   r         s         rho\n");
-          std::printf("------------------------------------\n");
+          std::printf("Itn   Fnc       DFid      Regl1     %sr         s"
+                      "         rho\n", grad ? "Regl2grd  " : "");
+          std::printf("%s\n", grad ? "-----------------------------------"
+                                     "-----------------------------------"
+                                   : "-----------------------------------"
+                                     "-------------------------");
         }
-        if (!ws->hctrl->done || ws->hctrl->iter == i)
-          std::printf("%4d %9.2e %9.2e %9.2e\n", ws->hctrl->iter,
-                      ws->hctrl->r, ws->hctrl->s, ws->hctrl->rho);
+        if (!ws->hctrl->done || ws->hctrl->iter == i) {
+          std::printf("%4d %9.2e %9.2e %9.2e ", ws->hctrl->iter, ov[0],
+                      ov[1], ov[2]);
+          if (grad) std::printf("%9.2e ", ov[3]);
+          std::printf("%9.2e %9.2e %9.2e\n", ws->hctrl->r, ws->hctrl->s,
+                      ws->hctrl->rho);
+        }
       }
       if (ws->hctrl->done) break;
     }
@@ -560,6 +667,14 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
     g_info.setup_ms = ms;
   if (hipEventElapsedTime(&ms, ws->ev[1], ws->ev[2]) == hipSuccess)
     g_info.solve_ms = ms;
+  if (obj_on) {
+    g_obj.resize((size_t)4 * cf.iter);
+    if (cf.iter > 0)
+      SCC_HIP(hipMemcpy(g_obj.data(), ws->oHist.p, g_obj.size() * 4,
+                        hipMemcpyDeviceToHost));
+  } else {
+    g_obj.clear();
+  }
   if (p.track) {
     g_hist.resize((size_t)3 * cf.iter);
     if (cf.iter > 0)
@@ -713,6 +828,15 @@ int scc_last_info(scc_info* info) {
 const char* scc_last_error(void) { return g_err.c_str(); }
 
 void scc_set_history(int enabled) { g_hist_on = enabled != 0; }
+
+void scc_set_objective(int enabled) { g_obj_on = enabled != 0; }
+
+int scc_last_objective(float* rows, int cap) {
+  const int n = (int)(g_obj.size() / 4);
+  if (rows != nullptr && cap > 0)
+    std::memcpy(rows, g_obj.data(), (size_t)std::min(n, cap) * 4 * 4);
+  return n;
+}
 
 int scc_last_history(float* rsr, int cap) {
   const int n = (int)(g_hist.size() / 3);

@@ -130,6 +130,25 @@ hipError_t launch_reduce(const Buffers& b, const Params& p, int nblocks,
                          int iter, int max_iter, hipStream_t st);
 int row_blocks(const Geom& g);
 
+// Objective kernels (objective.hip): per-frequency fidelity / gradient
+// sums over the spectra G of the objective variable (E != nullptr: store the
+// residual spectrum instead), L1 sums over the variable Z, masked spatial
+// fidelity, and the totals (Fnc, DFid, Regl1, Regl2g) into out / hist.
+int obj_blocks(const Geom& g);
+hipError_t launch_obj_freq(const Geom& g, const Buffers& b, const Params& p,
+                           const float2* G, float2* E, double* part, int iter,
+                           hipStream_t st);
+hipError_t launch_obj_l1(const Geom& g, const Params& p, const float* Z,
+                         const float* l1w, const Ctrl* ctrl, double* part,
+                         int nblk, int iter, hipStream_t st);
+hipError_t launch_obj_mask(const Geom& g, const float* e, const int* mask,
+                           const Ctrl* ctrl, double* part, int nblk, int iter,
+                           hipStream_t st);
+hipError_t launch_obj_final(const double* pf, int nf, const double* pl,
+                            int nl, const double* pm, int nm, const Params& p,
+                            double invN, const Ctrl* ctrl, float* out,
+                            float* hist, int iter, hipStream_t st);
+
 // Host helpers of the driver (api.hip): the thread's last-error message
 // returned by scc_last_error, and the mixed-radix plan of one FFT length.
 int set_error(int code, const std::string& msg);

@@ -90,6 +90,8 @@ def lib():
         'scc_last_error': (ctypes.c_char_p, []),
         'scc_set_history': (None, [_i]),
         'scc_last_history': (_i, [vp, _i]),
+        'scc_set_objective': (None, [_i]),
+        'scc_last_objective': (_i, [vp, _i]),
         'scc_get_device_count': (_i, []),
         'scc_get_current_device': (_i, [_pi]),
         'scc_set_current_device': (_i, [_i]),

@@ -85,6 +85,8 @@ cdef extern from "sporco_amd.h":
     const char *scc_last_error()
     void scc_set_history(int enabled)
     int scc_last_history(float *rsr, int cap)
+    void scc_set_objective(int enabled)
+    int scc_last_objective(float *rows, int cap)
 
 
 cdef _raise(int status):
@@ -141,6 +143,25 @@ def last_history():
     cdef np.ndarray[DTYPE_t, ndim=2, mode="c"] h = np.zeros((max(n, 1), 3),
                                                             dtype=DTYPE)
     scc_last_history(&h[0, 0], n)
+    return h[:n]
+
+
+def set_objective(enabled):
+    """Evaluate the objective after every iteration of later solves on this
+    thread (it is always evaluated, and printed, with Verbose)."""
+    scc_set_objective(1 if enabled else 0)
+
+
+def last_objective():
+    """(iterations, 4) array of (Fnc, DFid, Regl1, Regl2grd) of the last
+    solve: DFid = 1/2 ||sum_m d_m * z_m - s||^2 (masked for cbpdnmsk /
+    cbpdngrdmsk), Regl1 = ||w z||_1, Regl2grd = 1/2 sum_m w^g_m ||grad
+    z_m||^2, Fnc = DFid + lambda Regl1 (+ mu Regl2grd), with z = X, or Y when
+    AuxVarObj is set (sporco's ConvBPDN / ConvBPDNGradReg objective)."""
+    n = scc_last_objective(NULL, 0)
+    cdef np.ndarray[DTYPE_t, ndim=2, mode="c"] h = np.zeros((max(n, 1), 4),
+                                                            dtype=DTYPE)
+    scc_last_objective(&h[0, 0], n)
     return h[:n]
 
 
