@@ -10,11 +10,66 @@
 
 namespace scc {
 
+// Division by a value that is fixed over a loop, without the integer
+// divide sequence: a float estimate of the quotient, corrected to the exact
+// one (the estimate is off by at most one for the operands used here,
+// a < 2^22).
+struct FastDiv {
+  int d;
+  float inv;
+  __device__ __forceinline__ explicit FastDiv(int dd)
+      : d(dd), inv(1.f / (float)dd) {}
+  __device__ __forceinline__ int div(int a) const {
+    int q = (int)((float)a * inv);
+    int r = a - q * d;
+    while (r < 0) {
+      --q;
+      r += d;
+    }
+    while (r >= d) {
+      ++q;
+      r -= d;
+    }
+    return q;
+  }
+};
+
+// Radix-RX butterfly j of a Stockham stage from registers: inputs
+// in[r nb] (r < RX) times the stage twiddles W_L^{k tstep r}, a length-RX
+// DFT with W_RX^{q r} = W_L^{(q r mod RX) nb}, outputs out[q Ns].
+template <int RX>
+__device__ __forceinline__ void small_dft(const float2* in, float2* out,
+                                          int nb, int Ns, int k, int tstep,
+                                          const float2* __restrict__ tw,
+                                          int st, float dir) {
+  float2 x[RX];
+#pragma unroll
+  for (int r = 0; r < RX; ++r) x[r] = in[r * nb];
+  if (k) {
+#pragma unroll
+    for (int r = 1; r < RX; ++r)
+      x[r] = cmul(x[r], twiddle(tw, k * tstep * r * st, dir));
+  }
+#pragma unroll
+  for (int q = 0; q < RX; ++q) {
+    float2 acc = x[0];
+#pragma unroll
+    for (int r = 1; r < RX; ++r)
+      acc = cadd(acc, cmul(x[r], twiddle(tw, ((q * r) % RX) * nb * st, dir)));
+    out[q * Ns] = acc;
+  }
+}
+
 // --------------------------------------------------------------------------
 // Mixed-radix Stockham FFT over nt transforms of length pl.L held in LDS at
 // A + t*ld.  Ping-pongs between A and B and returns the buffer holding the
 // result.  dir = +1 forward (W = exp(-2 pi i / L)), -1 inverse (unnormalised).
 // Every thread of the block must call it (it contains block barriers).
+// Radices 2 and 4 and the small odd primes 3, 5, 7 run one butterfly per
+// thread from registers; larger prime radices one output per thread (so
+// that a radix-p stage keeps all threads busy): output q of butterfly j is
+//   sum_r in[j + r nb] W_L^{r (k tstep + q nb)},  k = j mod Ns,
+// the stage twiddle and the DFT kernel folded into one table exponent.
 // --------------------------------------------------------------------------
 __device__ float2* fft_generic(float2* A, float2* B, int nt, int ld,
                                const FftPlan& pl,
@@ -27,49 +82,66 @@ __device__ float2* fft_generic(float2* A, float2* B, int nt, int ld,
     const int Rx = pl.radix[s];
     const int nb = L / Rx;
     const int tstep = L / (Ns * Rx);
-    const int total = nt * nb;
-    for (int idx = tid; idx < total; idx += nthr) {
-      const int t = idx / nb;
-      const int j = idx - t * nb;
-      const float2* in = A + t * ld;
-      float2* out = B + t * ld;
-      const int k = j % Ns;
-      const int ob = (j - k) * Rx + k;
-      if (Rx == 2) {
-        float2 x0 = in[j];
-        float2 x1 = in[j + nb];
-        if (k) x1 = cmul(x1, twiddle(tw, k * tstep * st, dir));
-        out[ob] = cadd(x0, x1);
-        out[ob + Ns] = csub(x0, x1);
-      } else if (Rx == 4) {
-        float2 x0 = in[j];
-        float2 x1 = in[j + nb];
-        float2 x2 = in[j + 2 * nb];
-        float2 x3 = in[j + 3 * nb];
-        if (k) {
-          x1 = cmul(x1, twiddle(tw, k * tstep * st, dir));
-          x2 = cmul(x2, twiddle(tw, 2 * k * tstep * st, dir));
-          x3 = cmul(x3, twiddle(tw, 3 * k * tstep * st, dir));
-        }
-        float2 s02 = cadd(x0, x2), d02 = csub(x0, x2);
-        float2 s13 = cadd(x1, x3), d13 = csub(x1, x3);
-        float2 jd = mul_mi(d13, dir);  // -i*dir*d13
-        out[ob] = cadd(s02, s13);
-        out[ob + Ns] = cadd(d02, jd);
-        out[ob + 2 * Ns] = csub(s02, s13);
-        out[ob + 3 * Ns] = csub(d02, jd);
-      } else {
-        for (int q = 0; q < Rx; ++q) {
-          const int cq = k * tstep + q * nb;  // < L
-          float2 acc = make_float2(0.f, 0.f);
-          int e = 0;
-          for (int r = 0; r < Rx; ++r) {
-            acc = cadd(acc, cmul(in[j + r * nb], twiddle(tw, e * st, dir)));
-            e += cq;
-            if (e >= L) e -= L;
+    const FastDiv dnb(nb), dns(Ns);
+    if (Rx <= 7) {
+      const int total = nt * nb;
+      for (int idx = tid; idx < total; idx += nthr) {
+        const int t = dnb.div(idx);
+        const int j = idx - t * nb;
+        const float2* in = A + t * ld + j;
+        const int k = j - dns.div(j) * Ns;
+        float2* out = B + t * ld + (j - k) * Rx + k;
+        if (Rx == 2) {
+          float2 x0 = in[0];
+          float2 x1 = in[nb];
+          if (k) x1 = cmul(x1, twiddle(tw, k * tstep * st, dir));
+          out[0] = cadd(x0, x1);
+          out[Ns] = csub(x0, x1);
+        } else if (Rx == 4) {
+          float2 x0 = in[0];
+          float2 x1 = in[nb];
+          float2 x2 = in[2 * nb];
+          float2 x3 = in[3 * nb];
+          if (k) {
+            x1 = cmul(x1, twiddle(tw, k * tstep * st, dir));
+            x2 = cmul(x2, twiddle(tw, 2 * k * tstep * st, dir));
+            x3 = cmul(x3, twiddle(tw, 3 * k * tstep * st, dir));
           }
-          out[ob + q * Ns] = acc;
+          float2 s02 = cadd(x0, x2), d02 = csub(x0, x2);
+          float2 s13 = cadd(x1, x3), d13 = csub(x1, x3);
+          float2 jd = mul_mi(d13, dir);  // -i*dir*d13
+          out[0] = cadd(s02, s13);
+          out[Ns] = cadd(d02, jd);
+          out[2 * Ns] = csub(s02, s13);
+          out[3 * Ns] = csub(d02, jd);
+        } else if (Rx == 3) {
+          small_dft<3>(in, out, nb, Ns, k, tstep, tw, st, dir);
+        } else if (Rx == 5) {
+          small_dft<5>(in, out, nb, Ns, k, tstep, tw, st, dir);
+        } else {
+          small_dft<7>(in, out, nb, Ns, k, tstep, tw, st, dir);
         }
+      }
+    } else {
+      const FastDiv dL(L);
+      const int total = nt * L;
+      for (int idx = tid; idx < total; idx += nthr) {
+        const int t = dL.div(idx);
+        const int rem = idx - t * L;
+        const int q = dnb.div(rem);
+        const int j = rem - q * nb;
+        const int k = j - dns.div(j) * Ns;
+        const float2* in = A + t * ld + j;
+        int c = k * tstep + q * nb;  // < L + nb
+        if (c >= L) c -= L;
+        float2 acc = in[0];
+        int e = c;
+        for (int r = 1; r < Rx; ++r) {
+          acc = cadd(acc, cmul(in[r * nb], twiddle(tw, e * st, dir)));
+          e += c;
+          if (e >= L) e -= L;
+        }
+        B[t * ld + (j - k) * Rx + k + q * Ns] = acc;
       }
     }
     __syncthreads();
@@ -84,26 +156,34 @@ __device__ float2* fft_generic(float2* A, float2* B, int nt, int ld,
 // Element i of the (nt rows x H columns) half-spectrum block of rows
 // row0..row0+nt-1, enumerated panel by panel so that consecutive i touch
 // consecutive addresses when the rows are consecutive rows of one map.
-__device__ inline void panel_index(const Geom& g, int nt, int i, int& t,
-                                   int& c) {
-  const int full = (g.P - 1) * nt * g.TC;
-  if (i < full) {
-    const int per = nt * g.TC;
-    const int p = i / per;
-    const int rem = i - p * per;
-    t = rem / g.TC;
-    c = p * g.TC + (rem - t * g.TC);
-  } else {
-    const int rem = i - full;
-    t = rem / g.wlast;
-    c = (g.P - 1) * g.TC + (rem - t * g.wlast);
+struct PanelIndex {
+  int full, per, TC, wlast, c0;
+  FastDiv dper, dtc, dwl;
+  __device__ __forceinline__ PanelIndex(const Geom& g, int nt)
+      : full((g.P - 1) * nt * g.TC), per(nt * g.TC), TC(g.TC),
+        wlast(g.wlast), c0((g.P - 1) * g.TC), dper(nt * g.TC), dtc(g.TC),
+        dwl(g.wlast) {}
+  __device__ __forceinline__ void operator()(int i, int& t, int& c) const {
+    if (i < full) {
+      const int p = dper.div(i);
+      const int rem = i - p * per;
+      t = dtc.div(rem);
+      c = p * TC + (rem - t * TC);
+    } else {
+      const int rem = i - full;
+      t = dwl.div(rem);
+      c = c0 + (rem - t * wlast);
+    }
   }
-}
+};
 
-__device__ inline long long spec_addr(const Geom& g, long long row, int c) {
-  const long long m = row / g.R;
-  const int r = (int)(row - m * g.R);
-  return m * g.Hc + panel_off(g, r, c);
+// Offset of (row, c) of the half spectra of consecutive maps (row counts
+// rows of all maps); dR divides by g.R.
+__device__ inline long long spec_addr(const Geom& g, const FastDiv& dR,
+                                      int row, int c) {
+  const int m = dR.div(row);
+  const int r = row - m * g.R;
+  return (long long)m * g.Hc + panel_off(g, r, c);
 }
 
 // Inverse-real-FFT pre-processing of one block of half spectra (in B) into
@@ -113,8 +193,9 @@ __device__ void c2r_prep(const Geom& g, const float2* B, float2* A, int nt,
                          int nthr) {
   if (g.even) {
     const int L = g.Lr;
+    const FastDiv dl(L);
     for (int i = tid; i < nt * L; i += nthr) {
-      const int t = i / L;
+      const int t = dl.div(i);
       const int k = i - t * L;
       float2 xk = B[t * ld + k];
       float2 xn = B[t * ld + L - k];
@@ -129,8 +210,9 @@ __device__ void c2r_prep(const Geom& g, const float2* B, float2* A, int nt,
     }
   } else {
     const int C = g.C;
+    const FastDiv dc(C);
     for (int i = tid; i < nt * C; i += nthr) {
-      const int t = i / C;
+      const int t = dc.div(i);
       const int k = i - t * C;
       float2 v = (k < g.H) ? B[t * ld + k] : cconj(B[t * ld + C - k]);
       if (k == 0) v.y = 0.f;
@@ -145,9 +227,11 @@ __device__ void r2c_store(const Geom& g, const float2* Z, int nt, int ld,
                           long long row0, float2* __restrict__ F,
                           const float2* __restrict__ twC, int tid, int nthr) {
   const int L = g.Lr;
+  const PanelIndex pix(g, nt);
+  const FastDiv dR(g.R);
   for (int i = tid; i < nt * g.H; i += nthr) {
     int t, k;
-    panel_index(g, nt, i, t, k);
+    pix(i, t, k);
     float2 X;
     if (g.even) {
       const float2 zk = Z[t * ld + (k == L ? 0 : k)];
@@ -158,17 +242,19 @@ __device__ void r2c_store(const Geom& g, const float2* Z, int nt, int ld,
     } else {
       X = Z[t * ld + k];
     }
-    F[spec_addr(g, row0 + t, k)] = X;
+    F[spec_addr(g, dR, (int)row0 + t, k)] = X;
   }
 }
 
 __device__ void spec_load(const Geom& g, const float2* __restrict__ F,
                           float2* B, int nt, int ld, long long row0, int tid,
                           int nthr) {
+  const PanelIndex pix(g, nt);
+  const FastDiv dR(g.R);
   for (int i = tid; i < nt * g.H; i += nthr) {
     int t, c;
-    panel_index(g, nt, i, t, c);
-    B[t * ld + c] = F[spec_addr(g, row0 + t, c)];
+    pix(i, t, c);
+    B[t * ld + c] = F[spec_addr(g, dR, (int)row0 + t, c)];
   }
 }
 
@@ -196,8 +282,9 @@ __device__ inline void row_fwd_body(const Geom& g, long long nrows,
   const int tid = threadIdx.x, nthr = blockDim.x;
   const long long row0 = blk * nr;
   const int nt = (int)min((long long)nr, nrows - row0);
+  const FastDiv dc(g.C);
   for (int i = tid; i < nt * g.C; i += nthr) {
-    const int t = i / g.C;
+    const int t = dc.div(i);
     const int n = i - t * g.C;
     const long long gi = (row0 + t) * g.C + n;
     float a;
@@ -232,15 +319,16 @@ __device__ inline void col_body(const Geom& g, const FftPlan& cp, float2* F,
   float2* base = F + m * g.Hc + (long long)p * g.TC * g.R;
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int n = g.R * w;
+  const FastDiv dw(w);
   for (int i = tid; i < n; i += nthr) {
-    const int r = i / w;
+    const int r = dw.div(i);
     const int c = i - r * w;
     A[c * ld + r] = base[i];
   }
   __syncthreads();
   const float2* Z = fft_generic(A, B, w, ld, cp, twR, dir, tid, nthr);
   for (int i = tid; i < n; i += nthr) {
-    const int r = i / w;
+    const int r = dw.div(i);
     const int c = i - r * w;
     base[i] = Z[c * ld + r];
   }
@@ -272,12 +360,13 @@ __device__ inline void row_body(const Geom& g, const FftPlan& rp,
   UpdateSums acc;
   const float invN = 1.0f / (float)g.N;
   const float lr = p.lambda / rho;
+  const FastDiv dc(g.C), dR(g.R);
   for (int i = tid; i < nt * g.C; i += nthr) {
-    const int t = i / g.C;
+    const int t = dc.div(i);
     const int n = i - t * g.C;
     const long long row = row0 + t;
-    const int m = (int)(row / g.R);
-    const int r = (int)(row - (long long)m * g.R);
+    const int m = dR.div((int)row);
+    const int r = (int)row - m * g.R;
     const float xv = g.even ? reinterpret_cast<const float*>(P + t * ld)[n]
                             : P[t * ld + n].x;
     const long long gi = row * g.C + n;
@@ -315,8 +404,9 @@ __device__ inline void row_inv_body(const Geom& g, long long nrows,
   c2r_prep(g, B, A, nt, ld, twC, tid, nthr);
   __syncthreads();
   const float2* P = fft_generic(A, B, nt, ld, rp, twC, -1.f, tid, nthr);
+  const FastDiv dc(g.C);
   for (int i = tid; i < nt * g.C; i += nthr) {
-    const int t = i / g.C;
+    const int t = dc.div(i);
     const int n = i - t * g.C;
     const float xv = g.even ? reinterpret_cast<const float*>(P + t * ld)[n]
                             : P[t * ld + n].x;
