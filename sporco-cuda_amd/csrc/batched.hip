@@ -76,6 +76,12 @@ __device__ inline int find_img(const int* pre, int n, int item) {
   return lo;
 }
 
+// Image of a work item, made wavefront-uniform so that the descriptor is
+// read with scalar loads.
+__device__ __forceinline__ int img_of(const BList& L, int item) {
+  return __builtin_amdgcn_readfirstlane(find_img(L.pre, L.nimg, item));
+}
+
 // Y, U (M maps) and the work spectra F of every image to zero; grid.y
 // indexes the image.
 __global__ void k_bzero(const BImg* ims) {
@@ -116,20 +122,24 @@ __global__ void k_bpad(const BImg* ims, const float* __restrict__ D, int Kh,
 __global__ __launch_bounds__(256) void k_brow_fwd(BList L, int sel) {
   extern __shared__ float2 lds[];
   for (int item = blockIdx.x; item < L.total; item += gridDim.x) {
-    const int i = find_img(L.pre, L.nimg, item);
+    const int i = img_of(L, item);
     const BImg& im = L.im[i];
+    // Geometry and plan by value: the stores below would otherwise force
+    // their fields to be reloaded inside the transform loops.
+    const Geom g = im.g;
+    const FftPlan rp = im.rp;
     const long long blk = item - L.pre[i];
-    const long long mr = (long long)im.g.M * im.g.R;
+    const long long mr = (long long)g.M * g.R;
     if (sel == kSelWork) {
       if (im.ctrl->done) continue;
-      row_fwd_body(im.g, mr, im.rp, kFwdYU, nullptr, im.Y, im.U, im.F,
-                   im.twC, im.ctrl->rsf, im.nr, blk, lds);
+      row_fwd_body(g, mr, rp, kFwdYU, nullptr, im.Y, im.U, im.F, im.twC,
+                   im.ctrl->rsf, im.nr, blk, lds);
     } else if (sel == kSelDict) {
-      row_fwd_body(im.g, mr, im.rp, kFwdReal, im.Y, nullptr, nullptr, im.Df,
+      row_fwd_body(g, mr, rp, kFwdReal, im.Y, nullptr, nullptr, im.Df,
                    im.twC, 1.f, im.nr, blk, lds);
     } else {
-      row_fwd_body(im.g, im.g.R, im.rp, kFwdReal, im.S, nullptr, nullptr,
-                   im.Sf, im.twC, 1.f, im.nr, blk, lds);
+      row_fwd_body(g, g.R, rp, kFwdReal, im.S, nullptr, nullptr, im.Sf,
+                   im.twC, 1.f, im.nr, blk, lds);
     }
     __syncthreads();  // LDS is reused by the next item
   }
@@ -139,15 +149,16 @@ __global__ __launch_bounds__(256) void k_brow_fwd(BList L, int sel) {
 __global__ __launch_bounds__(256) void k_bcol(BList L, int sel, float dir) {
   extern __shared__ float2 lds[];
   for (int item = blockIdx.x; item < L.total; item += gridDim.x) {
-    const int i = find_img(L.pre, L.nimg, item);
+    const int i = img_of(L, item);
     const BImg& im = L.im[i];
     if (sel == kSelWork && im.ctrl->done) continue;
+    const Geom g = im.g;
+    const FftPlan cp = im.cp;
     const int local = item - L.pre[i];
-    const int P = im.g.P;
-    const int m = local / P;
-    const int p = local - m * P;
+    const int m = local / g.P;
+    const int p = local - m * g.P;
     float2* F = sel == kSelDict ? im.Df : (sel == kSelImage ? im.Sf : im.F);
-    col_body(im.g, im.cp, F, im.twR, dir, p, m, lds);
+    col_body(g, cp, F, im.twR, dir, p, m, lds);
     __syncthreads();
   }
 }
@@ -166,16 +177,19 @@ __global__ __launch_bounds__(256) void k_bsolve(BList L) {
   const int gi = lane / FPW;
   const int fl = lane - gi * FPW;
   for (int item = blockIdx.x; item < L.total; item += gridDim.x) {
-    const int i = find_img(L.pre, L.nimg, item);
+    const int i = img_of(L, item);
     const BImg& im = L.im[i];
     if (im.ctrl->done) continue;
     const long long Hc = im.g.Hc;
     const int M = im.g.M;
+    const float2* __restrict__ Dfi = im.Df;
+    const float2* __restrict__ Sfi = im.Sf;
+    float2* Fi = im.F;
     const long long f = (long long)(item - L.pre[i]) * FPB + wv * FPW + fl;
     const bool valid = f < Hc;
     const float rho = im.ctrl->rho;
     const float irho = 1.f / rho;
-    const float2 sh = valid ? im.Sf[f] : make_float2(0.f, 0.f);
+    const float2 sh = valid ? Sfi[f] : make_float2(0.f, 0.f);
     float2 bm[MPT], dm[MPT];
     float2 num = make_float2(0.f, 0.f);
     float den = 0.f;
@@ -186,8 +200,8 @@ __global__ __launch_bounds__(256) void k_bsolve(BList L) {
       dm[k] = make_float2(0.f, 0.f);
       if (valid && m < M) {
         const long long o = (long long)m * Hc + f;
-        dm[k] = im.Df[o];
-        const float2 a = im.F[o];
+        dm[k] = Dfi[o];
+        const float2 a = Fi[o];
         bm[k] = cadd(cmul(cconj(dm[k]), sh), cscale(a, rho));
         num = cadd(num, cmul(dm[k], bm[k]));
         den += dm[k].x * dm[k].x + dm[k].y * dm[k].y;
@@ -204,7 +218,7 @@ __global__ __launch_bounds__(256) void k_bsolve(BList L) {
     for (int k = 0; k < MPT; ++k) {
       const int m = gi + G * k;
       if (valid && m < M)
-        im.F[(long long)m * Hc + f] =
+        Fi[(long long)m * Hc + f] =
             cscale(csub(bm[k], cmul(cconj(dm[k]), cb)), irho);
     }
   }
@@ -217,9 +231,11 @@ __global__ __launch_bounds__(256) void k_brow(BList L, Params p,
   extern __shared__ float2 lds[];
   __shared__ float red[5][4];
   for (int item = blockIdx.x; item < L.total; item += gridDim.x) {
-    const int i = find_img(L.pre, L.nimg, item);
+    const int i = img_of(L, item);
     const BImg& im = L.im[i];
     if (im.ctrl->done) continue;
+    const Geom g = im.g;
+    const FftPlan rp = im.rp;
     Buffers b{};
     b.Y = im.Y;
     b.U = im.U;
@@ -230,8 +246,7 @@ __global__ __launch_bounds__(256) void k_brow(BList L, Params p,
     b.twC = im.twC;
     Params q = p;
     q.lambda = im.lambda;
-    row_body(im.g, im.rp, b, q, mode, im.nr, item - L.pre[i], im.nrow, lds,
-             red);
+    row_body(g, rp, b, q, mode, im.nr, item - L.pre[i], im.nrow, lds, red);
     __syncthreads();
   }
 }

@@ -66,10 +66,10 @@ __device__ __forceinline__ void small_dft(const float2* in, float2* out,
 // result.  dir = +1 forward (W = exp(-2 pi i / L)), -1 inverse (unnormalised).
 // Every thread of the block must call it (it contains block barriers).
 // Radices 2 and 4 and the small odd primes 3, 5, 7 run one butterfly per
-// thread from registers; larger prime radices one output per thread (so
-// that a radix-p stage keeps all threads busy): output q of butterfly j is
-//   sum_r in[j + r nb] W_L^{r (k tstep + q nb)},  k = j mod Ns,
-// the stage twiddle and the DFT kernel folded into one table exponent.
+// thread from registers; larger prime radices spread the outputs of a
+// butterfly over several threads (so that a radix-p stage keeps all threads
+// busy): output q of butterfly j is
+//   sum_r (in[j + r nb] W_L^{r k tstep}) W_Rx^{q r},  k = j mod Ns.
 // --------------------------------------------------------------------------
 __device__ float2* fft_generic(float2* A, float2* B, int nt, int ld,
                                const FftPlan& pl,
@@ -123,25 +123,51 @@ __device__ float2* fft_generic(float2* A, float2* B, int nt, int ld,
         }
       }
     } else {
-      const FastDiv dL(L);
-      const int total = nt * L;
+      // Prime radix >= 11: a work item computes kQB outputs q0 .. q0 + kQB
+      // - 1 of butterfly j.  Each input is read once and multiplied by its
+      // stage twiddle W_L^{r k tstep}; the DFT phases W_Rx^{q r} advance by
+      // one complex product per input (W_Rx^q) and are re-read from the
+      // table every 16 inputs, which bounds their rounding drift.
+      constexpr int kQB = 4;
+      const int nq = (Rx + kQB - 1) / kQB;
+      const FastDiv dblk(nb * nq), drx(Rx);
+      const int total = nt * nb * nq;
       for (int idx = tid; idx < total; idx += nthr) {
-        const int t = dL.div(idx);
-        const int rem = idx - t * L;
-        const int q = dnb.div(rem);
-        const int j = rem - q * nb;
+        const int t = dblk.div(idx);
+        const int rem = idx - t * nb * nq;
+        const int qb = dnb.div(rem);
+        const int j = rem - qb * nb;
         const int k = j - dns.div(j) * Ns;
+        const int q0 = qb * kQB;
         const float2* in = A + t * ld + j;
-        int c = k * tstep + q * nb;  // < L + nb
-        if (c >= L) c -= L;
-        float2 acc = in[0];
-        int e = c;
-        for (int r = 1; r < Rx; ++r) {
-          acc = cadd(acc, cmul(in[r * nb], twiddle(tw, e * st, dir)));
-          e += c;
-          if (e >= L) e -= L;
+        float2 acc[kQB], z[kQB], stp[kQB];
+#pragma unroll
+        for (int q = 0; q < kQB; ++q) {
+          acc[q] = make_float2(0.f, 0.f);
+          z[q] = make_float2(1.f, 0.f);
+          const int qq = min(q0 + q, Rx - 1);
+          stp[q] = twiddle(tw, qq * nb * st, dir);
         }
-        B[t * ld + (j - k) * Rx + k + q * Ns] = acc;
+        for (int r = 0; r < Rx; ++r) {
+          float2 x = in[r * nb];
+          if (k && r) x = cmul(x, twiddle(tw, r * k * tstep * st, dir));
+#pragma unroll
+          for (int q = 0; q < kQB; ++q) acc[q] = cadd(acc[q], cmul(x, z[q]));
+          if (((r + 1) & 15) == 0) {
+#pragma unroll
+            for (int q = 0; q < kQB; ++q) {
+              const int e = min(q0 + q, Rx - 1) * (r + 1);
+              z[q] = twiddle(tw, (e - drx.div(e) * Rx) * nb * st, dir);
+            }
+          } else {
+#pragma unroll
+            for (int q = 0; q < kQB; ++q) z[q] = cmul(z[q], stp[q]);
+          }
+        }
+        float2* out = B + t * ld + (j - k) * Rx + k;
+#pragma unroll
+        for (int q = 0; q < kQB; ++q)
+          if (q0 + q < Rx) out[(q0 + q) * Ns] = acc[q];
       }
     }
     __syncthreads();
