@@ -245,7 +245,7 @@ __global__ __launch_bounds__(256) void k_solve_pow2(Geom g, Buffers b,
 // support bits of V (|V| > t, sign V) to b.tbits, 2 V itself to b.vout.
 template <int LR, int TC, bool GENERAL, int TAPE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
-    GENERAL ? 1 : (TAPE == 1 ? 3 : RowCfg<LR>::WAVES)))) void
+    GENERAL ? 1 : RowCfg<LR>::WAVES))) void
 k_row_pow2(Geom g, Buffers b, Params p, int mode) {
   using RC = RowCfg<LR>;
   constexpr int E = RC::E, T = RC::T, RPW = RC::RPW;
@@ -372,14 +372,14 @@ k_row_pow2(Geom g, Buffers b, Params p, int mode) {
       const float x0 = z[e].x * invN, x1 = z[e].y * invN;
       const float v0 = fmaf(a, x0, fmaf(a1, y2[e].x, u2[e].x));
       const float v1 = fmaf(a, x1, fmaf(a1, y2[e].y, u2[e].y));
-      const float q0 = fabsf(v0) - th, q1 = fabsf(v1) - th;
-      const float yn0 = copysignf(fmaxf(q0, 0.f), v0);
-      const float yn1 = copysignf(fmaxf(q1, 0.f), v1);
+      const float yn0 = copysignf(fmaxf(fabsf(v0) - th, 0.f), v0);
+      const float yn1 = copysignf(fmaxf(fabsf(v1) - th, 0.f), v1);
       const float un0 = v0 - yn0, un1 = v1 - yn1;
+      // Support |V| > t is Y != 0, and there sign V = sign Y.
       if (TAPE == 1)
-        tb[e / 8] |= ((unsigned)(q0 > 0.f) | ((unsigned)(v0 < 0.f) << 1) |
-                      ((unsigned)(q1 > 0.f) << 2) |
-                      ((unsigned)(v1 < 0.f) << 3))
+        tb[e / 8] |= ((unsigned)(yn0 != 0.f) | ((unsigned)(yn0 < 0.f) << 1) |
+                      ((unsigned)(yn1 != 0.f) << 2) |
+                      ((unsigned)(yn1 < 0.f) << 3))
                      << (4 * (e % 8));
       if (TAPE == 2)
         reinterpret_cast<float2*>(b.vout + (long long)row0 * C)[lo + T * e] =

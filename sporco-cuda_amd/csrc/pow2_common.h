@@ -28,8 +28,8 @@ struct RowCfg {
 };
 
 // Soft-threshold support tape of the unrolled solver: 4 bits per lane slot
-// e (one float pair v0, v1): bit 0 |v0| > t, bit 1 v0 < 0, bit 2 |v1| > t,
-// bit 3 v1 < 0.  Lane t of row `row` keeps TapeWords<E>::W words at
+// e (one float pair v0, v1): bit 0 |v0| > t, bit 1 v0 < 0 (meaningful where
+// bit 0 is set), bit 2 |v1| > t, bit 3 v1 < 0.  Lane t of row `row` keeps TapeWords<E>::W words at
 // bits[(row * T + t) * W + w]; slot e is in word e / 8 at bit 4 (e % 8).
 template <int E>
 struct TapeWords {

@@ -547,6 +547,7 @@ struct UWs {
   Pow2Config cfg{};
   UBuf F, Df, Sf, Y, U, part, bsum, ctrl, twR, twC;
   UBuf tXh, tbits, Yb, Ub, Dhb, Shb, lacc, tmp, lparts;
+  UBuf Fr;  // work spectra of the backward's recompute
   size_t nr = 0, nc = 0, nbits = 0;  // per-iteration tape sizes
 };
 
@@ -681,8 +682,8 @@ int setup(UWs* w, const float* D, int K, const float* S, float rho,
   return SCC_OK;
 }
 
-// One forward iteration over the batch; F holds the row spectra of Y - U.
-// With tape pointers the solve also writes X-hat to xtape and the row
+// One forward iteration over the batch; F (or Fw when given) holds the row
+// spectra of Y - U.  With tape pointers the solve also writes X-hat to xtape and the row
 // kernel the support bits of V to tbits (or V itself to vout).  The row
 // kernel reads (Y, U) from (Yin, Uin) when given and writes them to
 // (Yout, Uout) when given, else in place in the workspace.
@@ -694,8 +695,10 @@ struct IterIO {
 };
 
 int fwd_iter(UWs* w, const Params& p, float2* xtape, unsigned* tbits,
-             float* vout, const IterIO& io, hipStream_t st) {
+             float* vout, const IterIO& io, hipStream_t st,
+             float2* Fw = nullptr) {
   Buffers b = make_buffers(w);
+  if (Fw != nullptr) b.F = Fw;
   b.xtape = xtape;
   b.tbits = tbits;
   b.vout = vout;
@@ -841,44 +844,58 @@ int scc_unrolled_backward(const float* D, int M, int K, const float* S, int B,
   float* lparts = (float*)w->lparts.p;
   int slot = 0;
   int np = 0;
-  for (int s = c.nseg - 1; s >= 0; --s) {
-    const int it0 = s * ckpt_every;
-    const int cnt = std::min(n_iter, it0 + ckpt_every) - it0;
-    const bool last = s == c.nseg - 1;
-    auto xh = [&](int j) {
-      return last ? tail_xh(w, ck, c, j)
-                  : (float2*)w->tXh.p + (size_t)j * w->nc;
-    };
-    auto bits = [&](int j) {
-      return last ? tail_bits(w, ck, c, j)
-                  : (unsigned*)w->tbits.p + (size_t)j * w->nbits;
-    };
-    if (!last) {
-      // Recompute the segment from its checkpoint (read in place: rsf is 1,
-      // so the row transform of Y - U does not write U), recording the tape.
-      float* cy = ck + (size_t)s * 2 * nr;
-      UHIP(pow2_row_fwd(w->gbm, w->cfg, BM, kFwdYU, nullptr, cy, cy + nr,
-                        b.F, b.twC, b.ctrl, st));
-      for (int j = 0; j < cnt; ++j) {
-        IterIO io;
-        if (j == 0) {
-          io.Yin = cy;
-          io.Uin = cy + nr;
-        }
-        if ((rc = fwd_iter(w, p, xh(j), bits(j), nullptr, io, st))) return rc;
+  // Tape of iteration j of segment sg: the last segment's lives with the
+  // checkpoints, the others' in the workspace (recomputed one at a time).
+  auto seg_cnt = [&](int sg) {
+    return std::min(n_iter, (sg + 1) * ckpt_every) - sg * ckpt_every;
+  };
+  auto xh = [&](int sg, int j) {
+    return sg == c.nseg - 1 ? tail_xh(w, ck, c, j)
+                            : (float2*)w->tXh.p + (size_t)j * w->nc;
+  };
+  auto bits = [&](int sg, int j) {
+    return sg == c.nseg - 1 ? tail_bits(w, ck, c, j)
+                            : (unsigned*)w->tbits.p + (size_t)j * w->nbits;
+  };
+  // Recompute segment sg from its checkpoint (read in place: rsf is 1, so
+  // the row transform of Y - U does not write U) into its own spectra Fr,
+  // recording the tape.
+  auto recompute = [&](int sg) -> int {
+    float* cy = ck + (size_t)sg * 2 * nr;
+    float2* Fr = (float2*)w->Fr.p;
+    UHIP(pow2_row_fwd(w->gbm, w->cfg, BM, kFwdYU, nullptr, cy, cy + nr, Fr,
+                      b.twC, b.ctrl, st));
+    for (int j = 0; j < seg_cnt(sg); ++j) {
+      IterIO io;
+      if (j == 0) {
+        io.Yin = cy;
+        io.Uin = cy + nr;
       }
+      int r2;
+      if ((r2 = fwd_iter(w, p, xh(sg, j), bits(sg, j), nullptr, io, st, Fr)))
+        return r2;
     }
-    // dL/dY enters through the first shrink backward.
-    UHIP(adj_row1(w->gbm, w->cfg, s == c.nseg - 1 ? gY : nullptr, Yb, Ub,
-                  bits(cnt - 1), alpha, b.F, b.twC,
-                  lparts + (size_t)slot++ * nparts, &np, st));
-    for (int j = cnt - 1; j >= 0; --j) {
+    return SCC_OK;
+  };
+  if (c.nseg > 1 && (rc = w->Fr.get(w->nc * 8))) return rc;
+  // dL/dY enters through the first shrink backward.
+  UHIP(adj_row1(w->gbm, w->cfg, gY, Yb, Ub, bits(c.nseg - 1, seg_cnt(c.nseg - 1) - 1),
+                alpha, b.F, b.twC, lparts + (size_t)slot++ * nparts, &np, st));
+  for (int sg = c.nseg - 1; sg >= 0; --sg) {
+    for (int j = seg_cnt(sg) - 1; j >= 0; --j) {
       UHIP(pow2_col(g, w->cfg, BM, b.F, b.twR, 0, nullptr, st));
-      UHIP(adj_solve(g, B, b.F, xh(j), b.Df, b.Sf, Dhb, Shb, rho, st));
+      UHIP(adj_solve(g, B, b.F, xh(sg, j), b.Df, b.Sf, Dhb, Shb, rho, st));
       UHIP(pow2_col(g, w->cfg, BM, b.F, b.twR, 1, nullptr, st));
       if (j > 0) {
-        UHIP(adj_row21(w->gbm, w->cfg, b.F, Yb, Ub, bits(j - 1), alpha,
+        UHIP(adj_row21(w->gbm, w->cfg, b.F, Yb, Ub, bits(sg, j - 1), alpha,
                        b.twC, lparts + (size_t)slot++ * nparts, &np, st));
+      } else if (sg > 0) {
+        // The previous segment's tape, then the fused rows across the
+        // segment boundary.
+        if ((rc = recompute(sg - 1))) return rc;
+        UHIP(adj_row21(w->gbm, w->cfg, b.F, Yb, Ub,
+                       bits(sg - 1, seg_cnt(sg - 1) - 1), alpha, b.twC,
+                       lparts + (size_t)slot++ * nparts, &np, st));
       } else {
         UHIP(adj_row2(w->gbm, w->cfg, b.F, Yb, Ub, nullptr, b.twC, 0, st));
       }
