@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "../../include/sporco_amd.h"
+#include "fft_gen.h"
 #include "scc_internal.h"
 #include "scc_pow2.h"
 
@@ -50,13 +51,23 @@ bool make_plan(int L, int tw_stride, FftPlan* pl) {
     pl->radix[pl->nstages++] = r;
     return true;
   };
-  while (n % 4 == 0) {
-    if (!push(4)) return false;
-    n /= 4;
-  }
+  // Power-of-two part 2^a: radix-8 stages, the rest as one radix-4 or
+  // radix-2 stage (8 x 2 as 4 x 4); then the odd primes in ascending order.
+  int a = 0;
   while (n % 2 == 0) {
-    if (!push(2)) return false;
     n /= 2;
+    ++a;
+  }
+  const int e8 = a / 3;
+  const int rest = a % 3;
+  for (int i = 0; i < e8 - (rest == 1 && e8 > 0 ? 1 : 0); ++i)
+    if (!push(8)) return false;
+  if (rest == 1 && e8 > 0) {
+    if (!push(4) || !push(4)) return false;
+  } else if (rest == 1) {
+    if (!push(2)) return false;
+  } else if (rest == 2) {
+    if (!push(4)) return false;
   }
   for (int p = 3; n > 1; p += 2) {
     while (n % p == 0) {
@@ -85,7 +96,7 @@ int make_geom(int R, int C, int M, Geom* g, bool* pow2, Pow2Config* p2) {
   if (*pow2) {
     TC = p2->TC;
   } else {
-    int cap = 65536 / (2 * (R + 1) * 8);
+    int cap = 65536 / (2 * gen::gld(R) * 8);
     if (cap < 1) cap = 1;
     cap = std::min(cap, g->H);
     TC = 1;
@@ -275,9 +286,10 @@ int resolve(const scc_algopt* in, float lambda, int grad, Resolved* rs) {
     return fail(SCC_ERR_INVALID, "filters are larger than the image");
   if ((long long)R * C > (1LL << 30) || (long long)M * R * C > (1LL << 36))
     return fail(SCC_ERR_UNSUPPORTED, "problem too large");
-  // LDS bounds of the generic kernels: 2 (R+1) and 2 (Lr+1) complex values
-  // of one column / row must fit in 64 KB with the reduction scratch.
-  if (R > 4095 || (C % 2 == 0 ? C > 7678 : C > 3839))
+  // LDS bounds of the generic kernels: two padded buffers of one column
+  // (gld(R) complex values) must fit in 64 KiB, two of one row (gld(Lr))
+  // in 60 KiB.
+  if (!gen::sides_fit_lds(R, C))
     return fail(SCC_ERR_UNSUPPORTED, "image side too large");
   if (o.MaxMainIter < 0)
     return fail(SCC_ERR_INVALID, "MaxMainIter must be >= 0");

@@ -119,27 +119,36 @@ __global__ void k_bpad(const BImg* ims, const float* __restrict__ D, int Kh,
 
 // Forward row transforms: padded filters -> Df, images -> Sf, or (AutoRho)
 // Y - U/rsf -> F.
+template <bool WAVE>
 __global__ __launch_bounds__(256) void k_brow_fwd(BList L, int sel) {
   extern __shared__ float2 lds[];
   for (int item = blockIdx.x; item < L.total; item += gridDim.x) {
     const int i = img_of(L, item);
     const BImg& im = L.im[i];
-    // Geometry and plan by value: the stores below would otherwise force
-    // their fields to be reloaded inside the transform loops.
+    // Geometry by value: the stores below would otherwise force its fields
+    // to be reloaded inside the transform loops.  The plan is read once per
+    // stage and stays in memory (a local copy indexed by stage would live in
+    // scratch).
     const Geom g = im.g;
-    const FftPlan rp = im.rp;
+    const FftPlan& rp = im.rp;
     const long long blk = item - L.pre[i];
     const long long mr = (long long)g.M * g.R;
+    auto body = [&](long long nrows, int mode, const float* src, float* Y,
+                    float* U, float2* F, float rsf) {
+      if constexpr (WAVE)
+        row_fwd_body_wave(g, nrows, rp, mode, src, Y, U, F, im.twC, rsf,
+                          im.nr, blk, lds);
+      else
+        row_fwd_body(g, nrows, rp, mode, src, Y, U, F, im.twC, rsf, im.nr,
+                     blk, lds);
+    };
     if (sel == kSelWork) {
       if (im.ctrl->done) continue;
-      row_fwd_body(g, mr, rp, kFwdYU, nullptr, im.Y, im.U, im.F, im.twC,
-                   im.ctrl->rsf, im.nr, blk, lds);
+      body(mr, kFwdYU, nullptr, im.Y, im.U, im.F, im.ctrl->rsf);
     } else if (sel == kSelDict) {
-      row_fwd_body(g, mr, rp, kFwdReal, im.Y, nullptr, nullptr, im.Df,
-                   im.twC, 1.f, im.nr, blk, lds);
+      body(mr, kFwdReal, im.Y, nullptr, nullptr, im.Df, 1.f);
     } else {
-      row_fwd_body(g, g.R, rp, kFwdReal, im.S, nullptr, nullptr, im.Sf,
-                   im.twC, 1.f, im.nr, blk, lds);
+      body(g.R, kFwdReal, im.S, nullptr, nullptr, im.Sf, 1.f);
     }
     __syncthreads();  // LDS is reused by the next item
   }
@@ -153,7 +162,7 @@ __global__ __launch_bounds__(256) void k_bcol(BList L, int sel, float dir) {
     const BImg& im = L.im[i];
     if (sel == kSelWork && im.ctrl->done) continue;
     const Geom g = im.g;
-    const FftPlan cp = im.cp;
+    const FftPlan& cp = im.cp;
     const int local = item - L.pre[i];
     const int m = local / g.P;
     const int p = local - m * g.P;
@@ -226,6 +235,7 @@ __global__ __launch_bounds__(256) void k_bsolve(BList L) {
 
 // Row kernel items: inverse row FFT, relax + shrink + dual update, residual
 // partial sums, and in fused mode the forward row FFT of Y - U.
+template <bool WAVE>
 __global__ __launch_bounds__(256) void k_brow(BList L, Params p,
                                               const float* l1w, int mode) {
   extern __shared__ float2 lds[];
@@ -235,7 +245,7 @@ __global__ __launch_bounds__(256) void k_brow(BList L, Params p,
     const BImg& im = L.im[i];
     if (im.ctrl->done) continue;
     const Geom g = im.g;
-    const FftPlan rp = im.rp;
+    const FftPlan& rp = im.rp;
     Buffers b{};
     b.Y = im.Y;
     b.U = im.U;
@@ -246,7 +256,11 @@ __global__ __launch_bounds__(256) void k_brow(BList L, Params p,
     b.twC = im.twC;
     Params q = p;
     q.lambda = im.lambda;
-    row_body(g, rp, b, q, mode, im.nr, item - L.pre[i], im.nrow, lds, red);
+    if constexpr (WAVE)
+      row_body_wave(g, rp, b, q, mode, im.nr, item - L.pre[i], im.nrow, lds,
+                    red);
+    else
+      row_body(g, rp, b, q, mode, im.nr, item - L.pre[i], im.nrow, lds, red);
     __syncthreads();
   }
 }
@@ -323,7 +337,8 @@ struct BBuf {
 
 struct BCfg {
   int col_lds = 65536;
-  int row_lds = 61440;
+  int row_lds = 61440;  // block-wide row kernels
+  int rpw = 2;          // rows per wavefront, wavefront-per-row kernels
   int mpt = 8;
   int persistent = 1;
 };
@@ -343,6 +358,7 @@ struct BWs {
   std::vector<int> hpre;  // kNumLists x (nimg + 1)
   int total[kNumLists] = {};
   int lds_row = 0, lds_col = 0;
+  bool wave = false;  // wavefront-per-row kernels (every row fits)
   int grid[kNumLists] = {};
   int G = 1;
   BBuf U, F, Df, Sf, part, ctrl, tw, desc, pre, l1w;
@@ -395,22 +411,26 @@ int configure(BWs* w, const BCfg& c, hipStream_t st) {
   w->hpre.assign((size_t)kNumLists * (n + 1), 0);
   w->lds_row = 0;
   w->lds_col = 0;
+  w->wave = true;
+  for (int i = 0; i < n; ++i) w->wave = w->wave && row_wave_fits(w->h[i].g);
   const int fpb = 4 * (64 / w->G);
   for (int i = 0; i < n; ++i) {
     BImg& im = w->h[i];
     Geom& g = im.g;
     const int R = g.R;
-    int cap = c.col_lds / (2 * (R + 1) * 8);
+    int cap = c.col_lds / (2 * gen::gld(R) * 8);
     cap = std::max(1, std::min(cap, g.H));
     int TC = 1;
     while (TC * 2 <= cap) TC *= 2;
     g.TC = TC;
     g.P = (g.H + TC - 1) / TC;
     g.wlast = g.H - (g.P - 1) * TC;
-    im.nr = row_nr_budget(g, c.row_lds);
+    im.nr = w->wave ? kRowWaves * c.rpw : row_nr_budget(g, c.row_lds);
     im.nrow = ceil_div((long long)w->M * R, im.nr);
-    w->lds_row = std::max(w->lds_row, 2 * im.nr * (g.Lr + 1) * 8);
-    w->lds_col = std::max(w->lds_col, 2 * TC * (R + 1) * 8);
+    w->lds_row = std::max(w->lds_row,
+                          w->wave ? row_wave_lds(g)
+                                  : 2 * im.nr * gen::gld(g.Lr) * 8);
+    w->lds_col = std::max(w->lds_col, 2 * TC * gen::gld(R) * 8);
     const int cnt[kNumLists] = {im.nrow, ceil_div(R, im.nr), g.P * w->M, g.P,
                                 ceil_div(g.Hc, fpb)};
     for (int k = 0; k < kNumLists; ++k) {
@@ -432,7 +452,9 @@ int configure(BWs* w, const BCfg& c, hipStream_t st) {
       b = 1;
     return b;
   };
-  const int occ_row = occ((const void*)k_brow, w->lds_row);
+  const int occ_row =
+      occ(w->wave ? (const void*)k_brow<true> : (const void*)k_brow<false>,
+          w->lds_row);
   const int occ_col = occ((const void*)k_bcol, w->lds_col);
   for (int k = 0; k < kNumLists; ++k) {
     int g = w->total[k];
@@ -555,13 +577,23 @@ void launch_col_b(BWs* w, int list, int sel, float dir, hipStream_t st) {
 }
 
 void launch_row_fwd_b(BWs* w, int list, int sel, hipStream_t st) {
-  hipLaunchKernelGGL(k_brow_fwd, dim3(w->grid[list]), dim3(256), w->lds_row,
-                     st, list_of(w, list), sel);
+  if (w->wave)
+    hipLaunchKernelGGL(k_brow_fwd<true>, dim3(w->grid[list]), dim3(256),
+                       w->lds_row, st, list_of(w, list), sel);
+  else
+    hipLaunchKernelGGL(k_brow_fwd<false>, dim3(w->grid[list]), dim3(256),
+                       w->lds_row, st, list_of(w, list), sel);
 }
 
 void launch_row_b(BWs* w, const Params& p, int mode, hipStream_t st) {
-  hipLaunchKernelGGL(k_brow, dim3(w->grid[kLRowM]), dim3(256), w->lds_row, st,
-                     list_of(w, kLRowM), p, (const float*)w->l1w.p, mode);
+  if (w->wave)
+    hipLaunchKernelGGL(k_brow<true>, dim3(w->grid[kLRowM]), dim3(256),
+                       w->lds_row, st, list_of(w, kLRowM), p,
+                       (const float*)w->l1w.p, mode);
+  else
+    hipLaunchKernelGGL(k_brow<false>, dim3(w->grid[kLRowM]), dim3(256),
+                       w->lds_row, st, list_of(w, kLRowM), p,
+                       (const float*)w->l1w.p, mode);
 }
 
 // One ADMM iteration of every image (the single-image sequence of
@@ -642,16 +674,22 @@ int autotune(BWs* w, const Params& p0, hipStream_t st) {
     }
   }
   bt = 1e30f;
-  for (int lds : {15360, 30720, 61440}) {
+  // Rows per wavefront (wavefront-per-row kernels) or an LDS budget of 15,
+  // 30 or 60 KiB (block-wide kernels).
+  for (int v : {1, 2, 4}) {
     BCfg c = best;
-    c.row_lds = lds;
+    if (w->wave)
+      c.rpw = v;
+    else
+      c.row_lds = 15360 * v;
     if ((rc = configure(w, c, st)) || (rc = reset())) return rc;
     if ((rc = time_it(w, st, 3, [&] { launch_row_b(w, p, kRowFused, st); },
                       &t)))
       return rc;
     if (t < bt) {
       bt = t;
-      best.row_lds = lds;
+      best.rpw = c.rpw;
+      best.row_lds = c.row_lds;
     }
   }
   bt = 1e30f;
@@ -745,7 +783,7 @@ int scc_batched_cbpdn(const float* D, int M, int Kh, int Kw, int nimg,
       return set_error(SCC_ERR_INVALID,
                        "image " + std::to_string(i) +
                            ": sizes must be >= 1 and >= the filter size");
-    if (b.R > 4095 || (b.C % 2 == 0 ? b.C > 7678 : b.C > 3839) ||
+    if (!gen::sides_fit_lds(b.R, b.C) ||
         (long long)M * b.R * b.C > (1LL << 36))
       return set_error(SCC_ERR_UNSUPPORTED,
                        "image " + std::to_string(i) + " too large");
@@ -903,7 +941,7 @@ int scc_batched_cbpdn(const float* D, int M, int Kh, int Kw, int nimg,
     }
   float ms = 0.f;
   b_last.col_lds = w->cfg.col_lds;
-  b_last.row_lds = w->cfg.row_lds;
+  b_last.row_lds = w->lds_row;
   b_last.solve_lanes = w->G;
   b_last.solve_maps = w->cfg.mpt;
   b_last.persistent = w->cfg.persistent;

@@ -1,0 +1,298 @@
+// Mixed-radix Stockham FFT with a run-time plan, for the generic-size
+// kernels (any image size).  nt transforms of length L live in LDS at
+// A + t * ld, element i at A + t * ld + gpad(i); a stage reads buffer A and
+// writes buffer B (ping-pong).  Stage s of radix Rx with Ns = product of the
+// earlier radices runs nb = L / Rx butterflies per transform; butterfly j
+// (k = j mod Ns) reads elements j + r nb (r < Rx), multiplies them by the
+// stage twiddles W_L^{r k L / (Ns Rx)}, takes their length-Rx DFT and writes
+// output q to (j - k) Rx + k + q Ns.
+//
+// Butterflies: radix 2, 4, 8, 16 as split-radix register DFTs with constant
+// twiddles; odd primes 3 .. 13 as register DFTs with constant coefficients
+// (inputs paired as x_r +- x_{P-r}); larger primes as a direct DFT whose work
+// items compute kQB outputs of one butterfly with recurrent phases.
+//
+// The per-stage functions are __host__ __device__ and loop over work items
+// tid, tid + nthr, ..., so that the CPU check runs them for every item in
+// turn with the barriers between stages replaced by the end of the loop.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fft_reg.h"
+
+namespace scc {
+namespace gen {
+
+using reg::c_add;
+using reg::c_mul;
+using reg::c_sub;
+
+// Padded LDS index: one spare slot after every 16 complex values, so that
+// the strided butterfly writes spread over the banks.
+__host__ __device__ __forceinline__ constexpr int gpad(int i) {
+  return i + (i >> 4);
+}
+// Complex slots of one padded buffer holding elements 0 .. L (inclusive:
+// the half-spectrum rows of even widths have L + 1 entries).
+__host__ __device__ __forceinline__ constexpr int gld(int L) {
+  return L + (L >> 4) + 1;
+}
+
+// Image sides whose generic column and row transforms fit the LDS of one
+// block: two padded buffers of a column in 64 KiB, of a row in 60 KiB.
+__host__ __device__ inline bool sides_fit_lds(int R, int C) {
+  const int Lr = (C % 2 == 0) ? C / 2 : C;
+  return R >= 1 && C >= 1 && 16 * gld(R) <= 65536 && 16 * gld(Lr) <= 61440;
+}
+
+// Quotient by a divisor fixed over a loop, for operands below 2^22: the
+// float estimate is within one of the quotient, corrected once each way.
+struct GDiv {
+  int d;
+  float inv;
+  __host__ __device__ __forceinline__ explicit GDiv(int dd)
+      : d(dd), inv(1.f / (float)dd) {}
+  __host__ __device__ __forceinline__ int div(int a) const {
+    int q = (int)((float)a * inv);
+    const int r = a - q * d;
+    if (r < 0) --q;
+    if (r >= d) ++q;
+    return q;
+  }
+};
+
+// cos and sin of 2 pi k / P for the odd primes with register butterflies,
+// k = 1 .. (P - 1) / 2.
+__host__ __device__ constexpr float odd_cos(int P, int k) {
+  return P == 3    ? -0.5f
+         : P == 5  ? (k == 1 ? 0.30901699437494745f : -0.80901699437494734f)
+         : P == 7  ? (k == 1   ? 0.62348980185873359f
+                      : k == 2 ? -0.22252093395631434f
+                               : -0.90096886790241903f)
+         : P == 11 ? (k == 1   ? 0.84125353283118121f
+                      : k == 2 ? 0.41541501300188644f
+                      : k == 3 ? -0.142314838273285f
+                      : k == 4 ? -0.65486073394528499f
+                               : -0.95949297361449737f)
+                   : (k == 1   ? 0.88545602565320991f
+                      : k == 2 ? 0.56806474673115592f
+                      : k == 3 ? 0.12053668025532301f
+                      : k == 4 ? -0.35460488704253545f
+                      : k == 5 ? -0.74851074817110119f
+                               : -0.97094181742605201f);
+}
+__host__ __device__ constexpr float odd_sin(int P, int k) {
+  return P == 3    ? 0.86602540378443871f
+         : P == 5  ? (k == 1 ? 0.95105651629515353f : 0.58778525229247325f)
+         : P == 7  ? (k == 1   ? 0.7818314824680298f
+                      : k == 2 ? 0.97492791218182362f
+                               : 0.43388373911755823f)
+         : P == 11 ? (k == 1   ? 0.54064081745559756f
+                      : k == 2 ? 0.90963199535451833f
+                      : k == 3 ? 0.9898214418809328f
+                      : k == 4 ? 0.75574957435425827f
+                               : 0.28173255684142967f)
+                   : (k == 1   ? 0.46472317204376851f
+                      : k == 2 ? 0.82298386589365635f
+                      : k == 3 ? 0.99270887409805397f
+                      : k == 4 ? 0.93501624268541483f
+                      : k == 5 ? 0.66312265824079519f
+                               : 0.23931566428755768f);
+}
+
+// In-place length-P DFT, P odd prime <= 13, forward (DIR = 1:
+// W = exp(-2 pi i / P)) or unnormalised inverse (DIR = -1).  With
+// a_r = x_r + x_{P-r}, b_r = x_r - x_{P-r} and theta = 2 pi q r / P:
+//   X_q     = x_0 + sum_r a_r cos(theta) - DIR i sum_r b_r sin(theta)
+//   X_{P-q} = x_0 + sum_r a_r cos(theta) + DIR i sum_r b_r sin(theta).
+template <int P, int DIR>
+__host__ __device__ __forceinline__ void dft_odd(float2* x) {
+  constexpr int H = (P - 1) / 2;
+  float2 a[H], b[H];
+#pragma unroll
+  for (int r = 1; r <= H; ++r) {
+    a[r - 1] = c_add(x[r], x[P - r]);
+    b[r - 1] = c_sub(x[r], x[P - r]);
+  }
+  float2 y0 = x[0];
+#pragma unroll
+  for (int r = 0; r < H; ++r) y0 = c_add(y0, a[r]);
+#pragma unroll
+  for (int q = 1; q <= H; ++q) {
+    float2 t = x[0];
+    float2 u = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int r = 1; r <= H; ++r) {
+      const int e = (q * r) % P;
+      const int ef = e <= H ? e : P - e;
+      const float c = odd_cos(P, ef);
+      const float s = e <= H ? odd_sin(P, ef) : -odd_sin(P, ef);
+      t.x += c * a[r - 1].x;
+      t.y += c * a[r - 1].y;
+      u.x += s * b[r - 1].x;
+      u.y += s * b[r - 1].y;
+    }
+    // DIR i u = DIR (-u.y, u.x)
+    const float2 iu = make_float2(-(float)DIR * u.y, (float)DIR * u.x);
+    x[q] = c_sub(t, iu);
+    x[P - q] = c_add(t, iu);
+  }
+  x[0] = y0;
+}
+
+template <int RX, int DIR>
+__host__ __device__ __forceinline__ void bfly(float2* x) {
+  if constexpr (RX == 2 || RX == 4 || RX == 8 || RX == 16)
+    reg::Dft<RX, DIR>::run(x);
+  else
+    dft_odd<RX, DIR>(x);
+}
+
+// W_L^e from the table W_{L tws}^t, conjugated for the inverse.
+template <int DIR>
+__host__ __device__ __forceinline__ float2 tw_at(const float2* tw, int e,
+                                                  int tws) {
+  float2 w = tw[e * tws];
+  if (DIR < 0) w.y = -w.y;
+  return w;
+}
+
+// Shape of one stage.
+struct GStage {
+  int L, Rx, nb, Ns, tstep;
+  bool p2;  // Ns is a power of two: k = j & (Ns - 1)
+};
+
+__host__ __device__ __forceinline__ GStage make_stage(int L, int Rx, int Ns) {
+  GStage s;
+  s.L = L;
+  s.Rx = Rx;
+  s.nb = L / Rx;
+  s.Ns = Ns;
+  s.tstep = L / (Ns * Rx);
+  s.p2 = (Ns & (Ns - 1)) == 0;
+  return s;
+}
+
+// Register-butterfly stage (RX <= 16).
+template <int RX, int DIR>
+__host__ __device__ __forceinline__ void stage_reg(const float2* A, float2* B,
+                                                   int ld, int nt,
+                                                   const GStage& s,
+                                                   const float2* tw, int tws,
+                                                   int tid, int nthr) {
+  const int nb = s.nb, Ns = s.Ns;
+  const GDiv dnb(nb), dns(Ns);
+  const int total = nt * nb;
+#pragma unroll 1
+  for (int idx = tid; idx < total; idx += nthr) {
+    int t = 0, j = idx;
+    if (nt > 1) {
+      t = dnb.div(idx);
+      j = idx - t * nb;
+    }
+    const int k = s.p2 ? (j & (Ns - 1)) : j - dns.div(j) * Ns;
+    const float2* in = A + t * ld;
+    float2* out = B + t * ld;
+    float2 x[RX];
+#pragma unroll
+    for (int r = 0; r < RX; ++r) x[r] = in[gpad(j + r * nb)];
+    if (k) {
+      const int e1 = k * s.tstep;
+#pragma unroll
+      for (int r = 1; r < RX; ++r) x[r] = c_mul(x[r], tw_at<DIR>(tw, e1 * r, tws));
+    }
+    bfly<RX, DIR>(x);
+    const int ob = (j - k) * RX + k;
+#pragma unroll
+    for (int q = 0; q < RX; ++q) out[gpad(ob + q * Ns)] = x[q];
+  }
+}
+
+// Prime radix >= 17: a work item computes kQB outputs q0 .. q0 + kQB - 1 of
+// butterfly j.  The stage twiddle W_L^{r k tstep} and the DFT phases
+// W_Rx^{q r} advance by one complex product per input and are re-read from
+// the table every 16 inputs, which bounds their rounding drift.
+template <int DIR>
+__host__ __device__ __forceinline__ void stage_prime(const float2* A,
+                                                     float2* B, int ld, int nt,
+                                                     const GStage& s,
+                                                     const float2* tw, int tws,
+                                                     int tid, int nthr) {
+  constexpr int kQB = 4;
+  const int Rx = s.Rx, nb = s.nb, Ns = s.Ns;
+  const int nq = (Rx + kQB - 1) / kQB;
+  const int per = nb * nq;
+  const GDiv dper(per), dnb(nb), dns(Ns);
+  const int total = nt * per;
+#pragma unroll 1
+  for (int idx = tid; idx < total; idx += nthr) {
+    int t = 0, rem = idx;
+    if (nt > 1) {
+      t = dper.div(idx);
+      rem = idx - t * per;
+    }
+    const int qb = dnb.div(rem);
+    const int j = rem - qb * nb;
+    const int k = s.p2 ? (j & (Ns - 1)) : j - dns.div(j) * Ns;
+    const int q0 = qb * kQB;
+    const float2* in = A + t * ld;
+    const int e1 = k * s.tstep;  // stage twiddle exponent step (< L / Rx)
+    const float2 ws = tw_at<DIR>(tw, e1, tws);
+    float2 wr = make_float2(1.f, 0.f);
+    float2 acc[kQB], z[kQB], stp[kQB];
+    int qq[kQB];
+#pragma unroll
+    for (int q = 0; q < kQB; ++q) {
+      qq[q] = (q0 + q < Rx) ? q0 + q : Rx - 1;
+      acc[q] = make_float2(0.f, 0.f);
+      z[q] = make_float2(1.f, 0.f);
+      stp[q] = tw_at<DIR>(tw, qq[q] * nb, tws);
+    }
+    for (int r = 0; r < Rx; ++r) {
+      float2 x = in[gpad(j + r * nb)];
+      if (k) x = c_mul(x, wr);
+#pragma unroll
+      for (int q = 0; q < kQB; ++q) acc[q] = c_add(acc[q], c_mul(x, z[q]));
+      if (((r + 1) & 15) == 0) {
+        if (r + 1 < Rx) wr = tw_at<DIR>(tw, e1 * (r + 1), tws);
+#pragma unroll
+        for (int q = 0; q < kQB; ++q) {
+          z[q] = tw_at<DIR>(tw, ((qq[q] * (r + 1)) % Rx) * nb, tws);
+        }
+      } else {
+        wr = c_mul(wr, ws);
+#pragma unroll
+        for (int q = 0; q < kQB; ++q) z[q] = c_mul(z[q], stp[q]);
+      }
+    }
+    float2* out = B + t * ld;
+    const int ob = (j - k) * Rx + k;
+#pragma unroll
+    for (int q = 0; q < kQB; ++q)
+      if (q0 + q < Rx) out[gpad(ob + (q0 + q) * Ns)] = acc[q];
+  }
+}
+
+// One stage of radix Rx: A -> B.
+template <int DIR>
+__host__ __device__ inline void stage(const float2* A, float2* B, int ld,
+                                      int nt, const GStage& s,
+                                      const float2* tw, int tws, int tid,
+                                      int nthr) {
+  switch (s.Rx) {
+    case 2: stage_reg<2, DIR>(A, B, ld, nt, s, tw, tws, tid, nthr); break;
+    case 4: stage_reg<4, DIR>(A, B, ld, nt, s, tw, tws, tid, nthr); break;
+    case 8: stage_reg<8, DIR>(A, B, ld, nt, s, tw, tws, tid, nthr); break;
+    case 3: stage_reg<3, DIR>(A, B, ld, nt, s, tw, tws, tid, nthr); break;
+    case 5: stage_reg<5, DIR>(A, B, ld, nt, s, tw, tws, tid, nthr); break;
+    case 7: stage_reg<7, DIR>(A, B, ld, nt, s, tw, tws, tid, nthr); break;
+    case 11: stage_reg<11, DIR>(A, B, ld, nt, s, tw, tws, tid, nthr); break;
+    case 13: stage_reg<13, DIR>(A, B, ld, nt, s, tw, tws, tid, nthr); break;
+    default: stage_prime<DIR>(A, B, ld, nt, s, tw, tws, tid, nthr); break;
+  }
+}
+
+}  // namespace gen
+}  // namespace scc

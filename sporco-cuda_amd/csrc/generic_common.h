@@ -5,6 +5,7 @@
 // index and dynamic LDS passed in.
 #pragma once
 
+#include "fft_gen.h"
 #include "scc_internal.h"
 #include "scc_device.h"
 
@@ -34,149 +35,62 @@ struct FastDiv {
   }
 };
 
-// Radix-RX butterfly j of a Stockham stage from registers: inputs
-// in[r nb] (r < RX) times the stage twiddles W_L^{k tstep r}, a length-RX
-// DFT with W_RX^{q r} = W_L^{(q r mod RX) nb}, outputs out[q Ns].
-template <int RX>
-__device__ __forceinline__ void small_dft(const float2* in, float2* out,
-                                          int nb, int Ns, int k, int tstep,
-                                          const float2* __restrict__ tw,
-                                          int st, float dir) {
-  float2 x[RX];
-#pragma unroll
-  for (int r = 0; r < RX; ++r) x[r] = in[r * nb];
-  if (k) {
-#pragma unroll
-    for (int r = 1; r < RX; ++r)
-      x[r] = cmul(x[r], twiddle(tw, k * tstep * r * st, dir));
-  }
-#pragma unroll
-  for (int q = 0; q < RX; ++q) {
-    float2 acc = x[0];
-#pragma unroll
-    for (int r = 1; r < RX; ++r)
-      acc = cadd(acc, cmul(x[r], twiddle(tw, ((q * r) % RX) * nb * st, dir)));
-    out[q * Ns] = acc;
-  }
-}
-
 // --------------------------------------------------------------------------
 // Mixed-radix Stockham FFT over nt transforms of length pl.L held in LDS at
-// A + t*ld.  Ping-pongs between A and B and returns the buffer holding the
-// result.  dir = +1 forward (W = exp(-2 pi i / L)), -1 inverse (unnormalised).
-// Every thread of the block must call it (it contains block barriers).
-// Radices 2 and 4 and the small odd primes 3, 5, 7 run one butterfly per
-// thread from registers; larger prime radices spread the outputs of a
-// butterfly over several threads (so that a radix-p stage keeps all threads
-// busy): output q of butterfly j is
-//   sum_r (in[j + r nb] W_L^{r k tstep}) W_Rx^{q r},  k = j mod Ns.
+// A + t*ld (padded element index gen::gpad).  Ping-pongs between A and B and
+// returns the buffer holding the result.  DIR = +1 forward
+// (W = exp(-2 pi i / L)), -1 inverse (unnormalised).  fft_block: every
+// thread of the block calls it (block barriers between stages); fft_wave:
+// one transform per wavefront (nt = 1), wavefront ordering only.
 // --------------------------------------------------------------------------
-__device__ float2* fft_generic(float2* A, float2* B, int nt, int ld,
-                               const FftPlan& pl,
-                               const float2* __restrict__ tw, float dir,
-                               int tid, int nthr) {
-  const int L = pl.L;
-  const int st = pl.tw_stride;
+template <int DIR>
+__device__ float2* fft_block(float2* A, float2* B, int nt, int ld,
+                             const FftPlan& pl, const float2* __restrict__ tw,
+                             int tid, int nthr) {
   int Ns = 1;
   for (int s = 0; s < pl.nstages; ++s) {
-    const int Rx = pl.radix[s];
-    const int nb = L / Rx;
-    const int tstep = L / (Ns * Rx);
-    const FastDiv dnb(nb), dns(Ns);
-    if (Rx <= 7) {
-      const int total = nt * nb;
-      for (int idx = tid; idx < total; idx += nthr) {
-        const int t = dnb.div(idx);
-        const int j = idx - t * nb;
-        const float2* in = A + t * ld + j;
-        const int k = j - dns.div(j) * Ns;
-        float2* out = B + t * ld + (j - k) * Rx + k;
-        if (Rx == 2) {
-          float2 x0 = in[0];
-          float2 x1 = in[nb];
-          if (k) x1 = cmul(x1, twiddle(tw, k * tstep * st, dir));
-          out[0] = cadd(x0, x1);
-          out[Ns] = csub(x0, x1);
-        } else if (Rx == 4) {
-          float2 x0 = in[0];
-          float2 x1 = in[nb];
-          float2 x2 = in[2 * nb];
-          float2 x3 = in[3 * nb];
-          if (k) {
-            x1 = cmul(x1, twiddle(tw, k * tstep * st, dir));
-            x2 = cmul(x2, twiddle(tw, 2 * k * tstep * st, dir));
-            x3 = cmul(x3, twiddle(tw, 3 * k * tstep * st, dir));
-          }
-          float2 s02 = cadd(x0, x2), d02 = csub(x0, x2);
-          float2 s13 = cadd(x1, x3), d13 = csub(x1, x3);
-          float2 jd = mul_mi(d13, dir);  // -i*dir*d13
-          out[0] = cadd(s02, s13);
-          out[Ns] = cadd(d02, jd);
-          out[2 * Ns] = csub(s02, s13);
-          out[3 * Ns] = csub(d02, jd);
-        } else if (Rx == 3) {
-          small_dft<3>(in, out, nb, Ns, k, tstep, tw, st, dir);
-        } else if (Rx == 5) {
-          small_dft<5>(in, out, nb, Ns, k, tstep, tw, st, dir);
-        } else {
-          small_dft<7>(in, out, nb, Ns, k, tstep, tw, st, dir);
-        }
-      }
-    } else {
-      // Prime radix >= 11: a work item computes kQB outputs q0 .. q0 + kQB
-      // - 1 of butterfly j.  Each input is read once and multiplied by its
-      // stage twiddle W_L^{r k tstep}; the DFT phases W_Rx^{q r} advance by
-      // one complex product per input (W_Rx^q) and are re-read from the
-      // table every 16 inputs, which bounds their rounding drift.
-      constexpr int kQB = 4;
-      const int nq = (Rx + kQB - 1) / kQB;
-      const FastDiv dblk(nb * nq), drx(Rx);
-      const int total = nt * nb * nq;
-      for (int idx = tid; idx < total; idx += nthr) {
-        const int t = dblk.div(idx);
-        const int rem = idx - t * nb * nq;
-        const int qb = dnb.div(rem);
-        const int j = rem - qb * nb;
-        const int k = j - dns.div(j) * Ns;
-        const int q0 = qb * kQB;
-        const float2* in = A + t * ld + j;
-        float2 acc[kQB], z[kQB], stp[kQB];
-#pragma unroll
-        for (int q = 0; q < kQB; ++q) {
-          acc[q] = make_float2(0.f, 0.f);
-          z[q] = make_float2(1.f, 0.f);
-          const int qq = min(q0 + q, Rx - 1);
-          stp[q] = twiddle(tw, qq * nb * st, dir);
-        }
-        for (int r = 0; r < Rx; ++r) {
-          float2 x = in[r * nb];
-          if (k && r) x = cmul(x, twiddle(tw, r * k * tstep * st, dir));
-#pragma unroll
-          for (int q = 0; q < kQB; ++q) acc[q] = cadd(acc[q], cmul(x, z[q]));
-          if (((r + 1) & 15) == 0) {
-#pragma unroll
-            for (int q = 0; q < kQB; ++q) {
-              const int e = min(q0 + q, Rx - 1) * (r + 1);
-              z[q] = twiddle(tw, (e - drx.div(e) * Rx) * nb * st, dir);
-            }
-          } else {
-#pragma unroll
-            for (int q = 0; q < kQB; ++q) z[q] = cmul(z[q], stp[q]);
-          }
-        }
-        float2* out = B + t * ld + (j - k) * Rx + k;
-#pragma unroll
-        for (int q = 0; q < kQB; ++q)
-          if (q0 + q < Rx) out[(q0 + q) * Ns] = acc[q];
-      }
-    }
+    const gen::GStage st = gen::make_stage(pl.L, pl.radix[s], Ns);
+    gen::stage<DIR>(A, B, ld, nt, st, tw, pl.tw_stride, tid, nthr);
     __syncthreads();
     float2* tmp = A;
     A = B;
     B = tmp;
-    Ns *= Rx;
+    Ns *= pl.radix[s];
   }
   return A;
+}
+
+template <int DIR>
+__device__ float2* fft_wave(float2* A, float2* B, const FftPlan& pl,
+                            const float2* __restrict__ tw, int lane) {
+  int Ns = 1;
+  for (int s = 0; s < pl.nstages; ++s) {
+    const gen::GStage st = gen::make_stage(pl.L, pl.radix[s], Ns);
+    gen::stage<DIR>(A, B, 0, 1, st, tw, pl.tw_stride, lane, 64);
+    reg::wave_sync();
+    float2* tmp = A;
+    A = B;
+    B = tmp;
+    Ns *= pl.radix[s];
+  }
+  return A;
+}
+
+__device__ inline float2* fft_generic(float2* A, float2* B, int nt, int ld,
+                                      const FftPlan& pl,
+                                      const float2* __restrict__ tw, float dir,
+                                      int tid, int nthr) {
+  return dir > 0.f ? fft_block<1>(A, B, nt, ld, pl, tw, tid, nthr)
+                   : fft_block<-1>(A, B, nt, ld, pl, tw, tid, nthr);
+}
+
+// Real value n of a row held as complex pairs (even widths) in a padded
+// buffer.
+__device__ __forceinline__ float& real_at(float2* row, int n) {
+  return reinterpret_cast<float*>(row + gen::gpad(n >> 1))[n & 1];
+}
+__device__ __forceinline__ float real_at(const float2* row, int n) {
+  return reinterpret_cast<const float*>(row + gen::gpad(n >> 1))[n & 1];
 }
 
 // Element i of the (nt rows x H columns) half-spectrum block of rows
@@ -223,8 +137,8 @@ __device__ void c2r_prep(const Geom& g, const float2* B, float2* A, int nt,
     for (int i = tid; i < nt * L; i += nthr) {
       const int t = dl.div(i);
       const int k = i - t * L;
-      float2 xk = B[t * ld + k];
-      float2 xn = B[t * ld + L - k];
+      float2 xk = B[t * ld + gen::gpad(k)];
+      float2 xn = B[t * ld + gen::gpad(L - k)];
       if (k == 0) {  // DC and Nyquist are real (C2R ignores their imag)
         xk.y = 0.f;
         xn.y = 0.f;
@@ -232,7 +146,7 @@ __device__ void c2r_prep(const Geom& g, const float2* B, float2* A, int nt,
       xn.y = -xn.y;
       const float2 s = cadd(xk, xn);
       const float2 wd = cmul(cconj(twC[k]), csub(xk, xn));
-      A[t * ld + k] = make_float2(s.x - wd.y, s.y + wd.x);  // s + i*wd
+      A[t * ld + gen::gpad(k)] = make_float2(s.x - wd.y, s.y + wd.x);  // s + i*wd
     }
   } else {
     const int C = g.C;
@@ -240,9 +154,10 @@ __device__ void c2r_prep(const Geom& g, const float2* B, float2* A, int nt,
     for (int i = tid; i < nt * C; i += nthr) {
       const int t = dc.div(i);
       const int k = i - t * C;
-      float2 v = (k < g.H) ? B[t * ld + k] : cconj(B[t * ld + C - k]);
+      float2 v = (k < g.H) ? B[t * ld + gen::gpad(k)]
+                           : cconj(B[t * ld + gen::gpad(C - k)]);
       if (k == 0) v.y = 0.f;
-      A[t * ld + k] = v;
+      A[t * ld + gen::gpad(k)] = v;
     }
   }
 }
@@ -260,13 +175,13 @@ __device__ void r2c_store(const Geom& g, const float2* Z, int nt, int ld,
     pix(i, t, k);
     float2 X;
     if (g.even) {
-      const float2 zk = Z[t * ld + (k == L ? 0 : k)];
-      const float2 zn = cconj(Z[t * ld + (k == 0 ? 0 : L - k)]);
+      const float2 zk = Z[t * ld + gen::gpad(k == L ? 0 : k)];
+      const float2 zn = cconj(Z[t * ld + gen::gpad(k == 0 ? 0 : L - k)]);
       const float2 s = cadd(zk, zn);
       const float2 wd = cmul(twC[k], csub(zk, zn));
       X = make_float2(0.5f * (s.x + wd.y), 0.5f * (s.y - wd.x));
     } else {
-      X = Z[t * ld + k];
+      X = Z[t * ld + gen::gpad(k)];
     }
     F[spec_addr(g, dR, (int)row0 + t, k)] = X;
   }
@@ -280,14 +195,14 @@ __device__ void spec_load(const Geom& g, const float2* __restrict__ F,
   for (int i = tid; i < nt * g.H; i += nthr) {
     int t, c;
     pix(i, t, c);
-    B[t * ld + c] = F[spec_addr(g, dR, (int)row0 + t, c)];
+    B[t * ld + gen::gpad(c)] = F[spec_addr(g, dR, (int)row0 + t, c)];
   }
 }
 
 // Rows of one row-kernel block for an LDS budget of `budget` bytes (two
 // ping-pong buffers of Lr + 1 complex values per row), at most 16.
 inline int row_nr_budget(const Geom& g, int budget) {
-  const int ld = g.Lr + 1;
+  const int ld = gen::gld(g.Lr);
   int nr = budget / (2 * ld * 8);
   if (nr > 16) nr = 16;
   if (nr < 1) nr = 1;
@@ -302,7 +217,7 @@ __device__ inline void row_fwd_body(const Geom& g, long long nrows,
                                     float2* F, const float2* __restrict__ twC,
                                     float rsf, int nr, long long blk,
                                     float2* lds) {
-  const int ld = g.Lr + 1;
+  const int ld = gen::gld(g.Lr);
   float2* A = lds;
   float2* B = lds + nr * ld;
   const int tid = threadIdx.x, nthr = blockDim.x;
@@ -325,9 +240,9 @@ __device__ inline void row_fwd_body(const Geom& g, long long nrows,
       a = Y[gi] - u;
     }
     if (g.even)
-      reinterpret_cast<float*>(A + t * ld)[n] = a;
+      real_at(A + t * ld, n) = a;
     else
-      A[t * ld + n] = make_float2(a, 0.f);
+      A[t * ld + gen::gpad(n)] = make_float2(a, 0.f);
   }
   __syncthreads();
   const float2* Z = fft_generic(A, B, nt, ld, rp, twC, 1.f, tid, nthr);
@@ -339,7 +254,7 @@ __device__ inline void col_body(const Geom& g, const FftPlan& cp, float2* F,
                                 const float2* __restrict__ twR, float dir,
                                 int p, long long m, float2* lds) {
   const int w = (p == g.P - 1) ? g.wlast : g.TC;
-  const int ld = g.R + 1;
+  const int ld = gen::gld(g.R);
   float2* A = lds;
   float2* B = lds + w * ld;
   float2* base = F + m * g.Hc + (long long)p * g.TC * g.R;
@@ -349,14 +264,14 @@ __device__ inline void col_body(const Geom& g, const FftPlan& cp, float2* F,
   for (int i = tid; i < n; i += nthr) {
     const int r = dw.div(i);
     const int c = i - r * w;
-    A[c * ld + r] = base[i];
+    A[c * ld + gen::gpad(r)] = base[i];
   }
   __syncthreads();
   const float2* Z = fft_generic(A, B, w, ld, cp, twR, dir, tid, nthr);
   for (int i = tid; i < n; i += nthr) {
     const int r = dw.div(i);
     const int c = i - r * w;
-    base[i] = Z[c * ld + r];
+    base[i] = Z[c * ld + gen::gpad(r)];
   }
 }
 
@@ -367,7 +282,7 @@ __device__ inline void row_body(const Geom& g, const FftPlan& rp,
                                 const Buffers& b, const Params& p, int mode,
                                 int nr, long long blk, int nparts,
                                 float2* lds, float (*red)[4]) {
-  const int ld = g.Lr + 1;
+  const int ld = gen::gld(g.Lr);
   float2* A = lds;
   float2* B = lds + nr * ld;
   const int tid = threadIdx.x, nthr = blockDim.x;
@@ -393,15 +308,14 @@ __device__ inline void row_body(const Geom& g, const FftPlan& rp,
     const long long row = row0 + t;
     const int m = dR.div((int)row);
     const int r = (int)row - m * g.R;
-    const float xv = g.even ? reinterpret_cast<const float*>(P + t * ld)[n]
-                            : P[t * ld + n].x;
+    const float xv = g.even ? real_at(P + t * ld, n) : P[t * ld + gen::gpad(n)].x;
     const long long gi = row * g.C + n;
     float a = update_point(g, p, b, xv * invN, lr, m, r, n, gi, acc);
     if (mode == kRowFused) {
       if (g.even)
-        reinterpret_cast<float*>(P + t * ld)[n] = a;
+        real_at(P + t * ld, n) = a;
       else
-        P[t * ld + n] = make_float2(a, 0.f);
+        P[t * ld + gen::gpad(n)] = make_float2(a, 0.f);
     }
   }
   block_sums_store(acc, red, b.partials, nparts, (int)blk, tid);
@@ -412,6 +326,184 @@ __device__ inline void row_body(const Geom& g, const FftPlan& rp,
   }
 }
 
+// --------------------------------------------------------------------------
+// Wavefront-per-row bodies: the block's kRowWaves wavefronts take rows
+// blk*nr + wv, + kRowWaves, ... (nr rows per block), each in its own pair of
+// LDS buffers of gld(Lr) complex values, ordered by wavefront sync only.
+// Used when the two buffers of 4 wavefronts fit 64 KiB (gld(Lr) <= 1024).
+// --------------------------------------------------------------------------
+constexpr int kRowWaves = 4;
+
+__host__ __device__ inline bool row_wave_fits(const Geom& g) {
+  return gen::gld(g.Lr) <= 1024;
+}
+__host__ __device__ inline int row_wave_lds(const Geom& g) {
+  return kRowWaves * 2 * gen::gld(g.Lr) * 8;
+}
+
+// Offset of (r, c) in one map's panel layout; TC = 1 << tsh.
+__device__ __forceinline__ long long panel_off_sh(const Geom& g, int tsh,
+                                                  int r, int c) {
+  const int p = c >> tsh;
+  const int w = (p == g.P - 1) ? g.wlast : g.TC;
+  return ((long long)p * g.R << tsh) + (long long)r * w + (c - (p << tsh));
+}
+
+// Half spectrum row (m, r) of Fm -> complex input of the inverse row
+// transform in A (B receives the raw row).
+__device__ __forceinline__ void wave_c2r_load(const Geom& g, int tsh,
+                                              const float2* __restrict__ Fm,
+                                              int r, float2* A, float2* B,
+                                              const float2* __restrict__ twC,
+                                              int lane) {
+  for (int c = lane; c < g.H; c += 64)
+    B[gen::gpad(c)] = Fm[panel_off_sh(g, tsh, r, c)];
+  reg::wave_sync();
+  if (g.even) {
+    const int L = g.Lr;
+    for (int k = lane; k < L; k += 64) {
+      float2 xk = B[gen::gpad(k)];
+      float2 xn = B[gen::gpad(L - k)];
+      if (k == 0) {  // DC and Nyquist are real (C2R ignores their imag)
+        xk.y = 0.f;
+        xn.y = 0.f;
+      }
+      xn.y = -xn.y;
+      const float2 s = cadd(xk, xn);
+      const float2 wd = cmul(cconj(twC[k]), csub(xk, xn));
+      A[gen::gpad(k)] = make_float2(s.x - wd.y, s.y + wd.x);  // s + i*wd
+    }
+  } else {
+    const int C = g.C;
+    for (int k = lane; k < C; k += 64) {
+      float2 v = (k < g.H) ? B[gen::gpad(k)] : cconj(B[gen::gpad(C - k)]);
+      if (k == 0) v.y = 0.f;
+      A[gen::gpad(k)] = v;
+    }
+  }
+  reg::wave_sync();
+}
+
+// Forward-real-FFT post-processing of the length-Lr transform Z and store
+// of half spectrum row (m, r) to Fm.
+__device__ __forceinline__ void wave_r2c_store(const Geom& g, int tsh,
+                                               const float2* Z, float2* Fm,
+                                               int r,
+                                               const float2* __restrict__ twC,
+                                               int lane) {
+  const int L = g.Lr;
+  for (int k = lane; k < g.H; k += 64) {
+    float2 X;
+    if (g.even) {
+      const float2 zk = Z[gen::gpad(k == L ? 0 : k)];
+      const float2 zn = cconj(Z[gen::gpad(k == 0 ? 0 : L - k)]);
+      const float2 s = cadd(zk, zn);
+      const float2 wd = cmul(twC[k], csub(zk, zn));
+      X = make_float2(0.5f * (s.x + wd.y), 0.5f * (s.y - wd.x));
+    } else {
+      X = Z[gen::gpad(k)];
+    }
+    Fm[panel_off_sh(g, tsh, r, k)] = X;
+  }
+}
+
+// Row kernel body, one wavefront per row: inverse row FFT -> X, fused
+// update and residual partial sums (reduced over the block into
+// partials[j * nparts + blk]), then in fused mode the forward row FFT of
+// Y - U.
+__device__ inline void row_body_wave(const Geom& g, const FftPlan& rp,
+                                     const Buffers& b, const Params& p,
+                                     int mode, int nr, long long blk,
+                                     int nparts, float2* lds,
+                                     float (*red)[4]) {
+  const int ld = gen::gld(g.Lr);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  float2* A = lds + wv * 2 * ld;
+  float2* B = A + ld;
+  const long long nrows = (long long)g.M * g.R;
+  const long long row0 = blk * nr;
+  const long long rend = min(row0 + nr, nrows);
+  const float rho = b.ctrl->rho;
+  const float invN = 1.0f / (float)g.N;
+  const float lr = p.lambda / rho;
+  const int tsh = __ffs(g.TC) - 1;
+  const int C = g.C;
+  UpdateSums acc;
+  for (long long row = row0 + wv; row < rend; row += kRowWaves) {
+    const int m = (int)(row / g.R);
+    const int r = (int)(row - (long long)m * g.R);
+    float2* Fm = b.F + (long long)m * g.Hc;
+    wave_c2r_load(g, tsh, Fm, r, A, B, b.twC, lane);
+    float2* P = fft_wave<-1>(A, B, rp, b.twC, lane);
+    float2* Q = (P == A) ? B : A;
+    const long long gi0 = row * C;
+    for (int n = lane; n < C; n += 64) {
+      float* slot = g.even ? &real_at(P, n) : &P[gen::gpad(n)].x;
+      const float a = update_point(g, p, b, *slot * invN, lr, m, r, n,
+                                   gi0 + n, acc);
+      if (mode == kRowFused) {
+        if (g.even)
+          *slot = a;
+        else
+          P[gen::gpad(n)] = make_float2(a, 0.f);
+      }
+    }
+    if (mode == kRowFused) {
+      reg::wave_sync();
+      const float2* Z = fft_wave<1>(P, Q, rp, b.twC, lane);
+      wave_r2c_store(g, tsh, Z, Fm, r, b.twC, lane);
+    }
+    reg::wave_sync();  // this row's LDS reads before the next row's writes
+  }
+  block_sums_store(acc, red, b.partials, nparts, (int)blk, threadIdx.x);
+}
+
+// Forward row transform, one wavefront per row, of rows blk*nr.. of a real
+// array (setup) or of Y - U/rsf (AutoRho), into F.
+__device__ inline void row_fwd_body_wave(const Geom& g, long long nrows,
+                                         const FftPlan& rp, int mode,
+                                         const float* src, float* Y, float* U,
+                                         float2* F,
+                                         const float2* __restrict__ twC,
+                                         float rsf, int nr, long long blk,
+                                         float2* lds) {
+  const int ld = gen::gld(g.Lr);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  float2* A = lds + wv * 2 * ld;
+  float2* B = A + ld;
+  const long long row0 = blk * nr;
+  const long long rend = min(row0 + nr, nrows);
+  const int tsh = __ffs(g.TC) - 1;
+  const int C = g.C;
+  for (long long row = row0 + wv; row < rend; row += kRowWaves) {
+    const int m = (int)(row / g.R);
+    const int r = (int)(row - (long long)m * g.R);
+    const long long gi0 = row * C;
+    for (int n = lane; n < C; n += 64) {
+      const long long gi = gi0 + n;
+      float a;
+      if (mode == kFwdReal) {
+        a = src[gi];
+      } else {
+        float u = U[gi];
+        if (rsf != 1.f) {
+          u = u / rsf;
+          U[gi] = u;
+        }
+        a = Y[gi] - u;
+      }
+      if (g.even)
+        real_at(A, n) = a;
+      else
+        A[gen::gpad(n)] = make_float2(a, 0.f);
+    }
+    reg::wave_sync();
+    const float2* Z = fft_wave<1>(A, B, rp, twC, lane);
+    wave_r2c_store(g, tsh, Z, F + (long long)m * g.Hc, r, twC, lane);
+    reg::wave_sync();
+  }
+}
+
 // Inverse real row transform of rows blk*nr.. of the half spectra F into
 // out (real rows of length C), times scale.
 __device__ inline void row_inv_body(const Geom& g, long long nrows,
@@ -419,7 +511,7 @@ __device__ inline void row_inv_body(const Geom& g, long long nrows,
                                     float* out, const float2* __restrict__ twC,
                                     float scale, int nr, long long blk,
                                     float2* lds) {
-  const int ld = g.Lr + 1;
+  const int ld = gen::gld(g.Lr);
   float2* A = lds;
   float2* B = lds + nr * ld;
   const int tid = threadIdx.x, nthr = blockDim.x;
@@ -434,8 +526,7 @@ __device__ inline void row_inv_body(const Geom& g, long long nrows,
   for (int i = tid; i < nt * g.C; i += nthr) {
     const int t = dc.div(i);
     const int n = i - t * g.C;
-    const float xv = g.even ? reinterpret_cast<const float*>(P + t * ld)[n]
-                            : P[t * ld + n].x;
+    const float xv = g.even ? real_at(P + t * ld, n) : P[t * ld + gen::gpad(n)].x;
     out[(row0 + t) * g.C + n] = xv * scale;
   }
 }

@@ -18,7 +18,19 @@
 
 namespace scc {
 
-static inline int row_nr(const Geom& g) { return row_nr_budget(g, 61440); }
+// Rows per row-kernel block: kRowWaves wavefronts of one or two rows each
+// (wavefront-per-row kernels), or as many as fit a 60 KiB LDS budget
+// (block-wide kernels, rows longer than the wavefront buffers allow).
+static inline int row_nr(const Geom& g) {
+  if (!row_wave_fits(g)) return row_nr_budget(g, 61440);
+  const long long nrows = (long long)g.M * g.R;
+  return nrows >= 2LL * kRowWaves * 2048 ? 2 * kRowWaves : kRowWaves;
+}
+
+static inline size_t row_lds(const Geom& g, int nr) {
+  return row_wave_fits(g) ? (size_t)row_wave_lds(g)
+                          : (size_t)2 * nr * gen::gld(g.Lr) * sizeof(float2);
+}
 
 // --------------------------------------------------------------------------
 // Filter padding (cuda_Pad_Dict, cbpdn_kernels.cu:602-615): origin at [0,0].
@@ -39,6 +51,7 @@ __global__ void k_pad(const float* __restrict__ D, float* __restrict__ out,
 // --------------------------------------------------------------------------
 // Forward row transform of a real array (setup) or of Y - U/rsf (AutoRho).
 // --------------------------------------------------------------------------
+template <bool WAVE>
 __global__ __launch_bounds__(256) void k_row_fwd_generic(
     Geom g, long long nrows, FftPlan rp, int mode, const float* src,
     float* Y, float* U, float2* F, const float2* __restrict__ twC,
@@ -46,8 +59,12 @@ __global__ __launch_bounds__(256) void k_row_fwd_generic(
   if (mode == kFwdYU && ctrl->done) return;
   extern __shared__ float2 lds[];
   const float rsf = (mode == kFwdYU) ? ctrl->rsf : 1.f;
-  row_fwd_body(g, nrows, rp, mode, src, Y, U, F, twC, rsf, nr, blockIdx.x,
-               lds);
+  if constexpr (WAVE)
+    row_fwd_body_wave(g, nrows, rp, mode, src, Y, U, F, twC, rsf, nr,
+                      blockIdx.x, lds);
+  else
+    row_fwd_body(g, nrows, rp, mode, src, Y, U, F, twC, rsf, nr, blockIdx.x,
+                 lds);
 }
 
 // Inverse real row transform of half spectra into real rows, times scale.
@@ -120,13 +137,17 @@ __global__ __launch_bounds__(256) void k_solve_generic(Geom g, Buffers b,
 // Row kernel: inverse row FFT -> X, fused relax + shrink + AMS + dual update
 // + residual partial sums, then (fused mode) forward row FFT of Y - U.
 // --------------------------------------------------------------------------
+template <bool WAVE>
 __global__ __launch_bounds__(256) void k_row_generic(Geom g, FftPlan rp,
                                                       Buffers b, Params p,
                                                       int mode, int nr) {
   if (b.ctrl->done) return;
   extern __shared__ float2 lds[];
   __shared__ float red[5][4];
-  row_body(g, rp, b, p, mode, nr, blockIdx.x, gridDim.x, lds, red);
+  if constexpr (WAVE)
+    row_body_wave(g, rp, b, p, mode, nr, blockIdx.x, gridDim.x, lds, red);
+  else
+    row_body(g, rp, b, p, mode, nr, blockIdx.x, gridDim.x, lds, red);
 }
 
 // --------------------------------------------------------------------------
@@ -208,19 +229,23 @@ hipError_t launch_row_fwd(const Geom& g, int nmaps, const FftPlan& rp,
   const int nr = row_nr(g);
   const long long nrows = (long long)nmaps * g.R;
   const unsigned nb = (unsigned)((nrows + nr - 1) / nr);
-  const size_t shm = (size_t)2 * nr * (g.Lr + 1) * sizeof(float2);
-  hipLaunchKernelGGL(k_row_fwd_generic, dim3(nb), dim3(256), shm, st, g,
-                     nrows, rp, mode, src, Y, U, F, twC, ctrl, nr);
+  const size_t shm = row_lds(g, nr);
+  if (row_wave_fits(g))
+    hipLaunchKernelGGL(k_row_fwd_generic<true>, dim3(nb), dim3(256), shm, st,
+                       g, nrows, rp, mode, src, Y, U, F, twC, ctrl, nr);
+  else
+    hipLaunchKernelGGL(k_row_fwd_generic<false>, dim3(nb), dim3(256), shm, st,
+                       g, nrows, rp, mode, src, Y, U, F, twC, ctrl, nr);
   return hipGetLastError();
 }
 
 hipError_t launch_row_inv(const Geom& g, int nmaps, const FftPlan& rp,
                           float2* F, float* out, const float2* twC,
                           float scale, hipStream_t st) {
-  const int nr = row_nr(g);
+  const int nr = row_nr_budget(g, 61440);
   const long long nrows = (long long)nmaps * g.R;
   const unsigned nb = (unsigned)((nrows + nr - 1) / nr);
-  const size_t shm = (size_t)2 * nr * (g.Lr + 1) * sizeof(float2);
+  const size_t shm = (size_t)2 * nr * gen::gld(g.Lr) * sizeof(float2);
   hipLaunchKernelGGL(k_row_inv_generic, dim3(nb), dim3(256), shm, st, g,
                      nrows, rp, F, out, twC, scale, nr);
   return hipGetLastError();
@@ -229,7 +254,7 @@ hipError_t launch_row_inv(const Geom& g, int nmaps, const FftPlan& rp,
 hipError_t launch_col(const Geom& g, int nmaps, const FftPlan& cp, float2* F,
                       const float2* twR, int inverse, const Ctrl* ctrl,
                       hipStream_t st) {
-  const size_t shm = (size_t)2 * g.TC * (g.R + 1) * sizeof(float2);
+  const size_t shm = (size_t)2 * g.TC * gen::gld(g.R) * sizeof(float2);
   hipLaunchKernelGGL(k_col_generic, dim3(g.P, nmaps), dim3(256), shm, st, g,
                      cp, F, twR, inverse ? -1.f : 1.f, ctrl);
   return hipGetLastError();
@@ -247,9 +272,13 @@ hipError_t launch_row(const Geom& g, const FftPlan& rp, const Buffers& b,
                       hipStream_t st) {
   const int nr = row_nr(g);
   const int nb = row_blocks(g);
-  const size_t shm = (size_t)2 * nr * (g.Lr + 1) * sizeof(float2);
-  hipLaunchKernelGGL(k_row_generic, dim3(nb), dim3(256), shm, st, g, rp, b,
-                     p, mode, nr);
+  const size_t shm = row_lds(g, nr);
+  if (row_wave_fits(g))
+    hipLaunchKernelGGL(k_row_generic<true>, dim3(nb), dim3(256), shm, st, g,
+                       rp, b, p, mode, nr);
+  else
+    hipLaunchKernelGGL(k_row_generic<false>, dim3(nb), dim3(256), shm, st, g,
+                       rp, b, p, mode, nr);
   nblocks_out[0] = nb;
   return hipGetLastError();
 }

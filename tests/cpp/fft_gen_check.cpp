@@ -1,0 +1,121 @@
+// CPU check of the run-time-plan Stockham FFT stages (csrc/fft_gen.h): runs
+// gen::stage for every work item in turn (the barriers between stages
+// become the end of the item loop), for one transform per wavefront
+// (nt = 1, 64 items at a time) and several per block (nt > 1, 256), and
+// compares the result with a direct double-precision DFT.  Also checks the
+// float-based divider against integer division.  Prints one line per case
+// and exits non-zero on failure.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "fft_gen.h"
+
+using namespace scc::gen;
+
+static int check(int L, const std::vector<int>& radices, int tws, int dir,
+                 int nt, int nthr) {
+  int prod = 1;
+  for (int r : radices) prod *= r;
+  if (prod != L) {
+    printf("bad plan for L=%d\n", L);
+    return 1;
+  }
+  const int Lt = L * tws;
+  std::vector<float2> tw(Lt);
+  for (int t = 0; t < Lt; ++t) {
+    const double a = -2.0 * M_PI * t / Lt;
+    tw[t] = make_float2((float)cos(a), (float)sin(a));
+  }
+  const int ld = gld(L) + (nt > 1 ? 3 : 0);
+  std::vector<float2> A(nt * ld, make_float2(1e30f, 1e30f)), B = A;
+  std::vector<double> xr(nt * L), xi(nt * L);
+  srand(L * 131 + nt * 7 + (dir > 0) + tws * 3);
+  for (int t = 0; t < nt; ++t)
+    for (int n = 0; n < L; ++n) {
+      xr[t * L + n] = rand() / (double)RAND_MAX - 0.5;
+      xi[t * L + n] = rand() / (double)RAND_MAX - 0.5;
+      A[t * ld + gpad(n)] =
+          make_float2((float)xr[t * L + n], (float)xi[t * L + n]);
+    }
+  float2* a = A.data();
+  float2* b = B.data();
+  int Ns = 1;
+  for (int r : radices) {
+    const GStage st = make_stage(L, r, Ns);
+    for (int tid = 0; tid < nthr; ++tid) {
+      if (dir > 0)
+        stage<1>(a, b, ld, nt, st, tw.data(), tws, tid, nthr);
+      else
+        stage<-1>(a, b, ld, nt, st, tw.data(), tws, tid, nthr);
+    }
+    float2* tmp = a;
+    a = b;
+    b = tmp;
+    Ns *= r;
+  }
+  double err = 0, ref = 0;
+  for (int t = 0; t < nt; ++t)
+    for (int k = 0; k < L; ++k) {
+      double sr = 0, si = 0;
+      for (int n = 0; n < L; ++n) {
+        const double ang =
+            -2.0 * M_PI * dir * (double)((long long)n * k % L) / L;
+        sr += xr[t * L + n] * cos(ang) - xi[t * L + n] * sin(ang);
+        si += xr[t * L + n] * sin(ang) + xi[t * L + n] * cos(ang);
+      }
+      const float2 g = a[t * ld + gpad(k)];
+      err = fmax(err, fabs(g.x - sr) + fabs(g.y - si));
+      ref = fmax(ref, fabs(sr) + fabs(si));
+    }
+  const double rel = err / ref;
+  const bool ok = rel < 3e-6;
+  printf("L=%d plan=", L);
+  for (size_t i = 0; i < radices.size(); ++i)
+    printf("%s%d", i ? "x" : "", radices[i]);
+  printf(" dir=%d tws=%d nt=%d nthr=%d rel_err=%.3e %s\n", dir, tws, nt, nthr,
+         rel, ok ? "ok" : "FAIL");
+  return ok ? 0 : 1;
+}
+
+static int check_div() {
+  int bad = 0;
+  for (int d = 1; d <= 4100; d += (d < 64 ? 1 : 13)) {
+    const GDiv q(d);
+    for (int a = 0; a < (1 << 22); a += (a < 70000 ? 1 : 9973))
+      if (q.div(a) != a / d) {
+        if (bad < 5) printf("GDiv(%d).div(%d) = %d\n", d, a, q.div(a));
+        ++bad;
+      }
+    const int a = (1 << 22) - 1;
+    if (q.div(a) != a / d) ++bad;
+  }
+  printf("GDiv %s\n", bad ? "FAIL" : "ok");
+  return bad ? 1 : 0;
+}
+
+int main() {
+  int bad = check_div();
+  struct Case {
+    int L;
+    std::vector<int> r;
+    int tws;
+  };
+  const std::vector<Case> cases = {
+      {384, {8, 8, 2, 3}, 2},   {384, {8, 4, 4, 3}, 2}, {356, {4, 89}, 2},
+      {64, {8, 8}, 1},          {96, {8, 4, 3}, 1},     {128, {8, 4, 4}, 1},
+      {2, {2}, 1},              {4, {4}, 2},            {48, {16, 3}, 1},
+      {45, {3, 3, 5}, 1},       {105, {3, 5, 7}, 1},    {143, {11, 13}, 1},
+      {264, {8, 3, 11}, 2},     {17, {17}, 1},          {31, {31}, 1},
+      {232, {8, 29}, 2},        {712, {8, 89}, 1},      {664, {8, 83}, 2},
+      {184, {8, 23}, 1},        {97, {97}, 1},          {200, {8, 5, 5}, 2},
+      {1, {}, 1},
+  };
+  for (const Case& c : cases)
+    for (int dir : {1, -1}) {
+      bad += check(c.L, c.r, c.tws, dir, 1, 64);
+      bad += check(c.L, c.r, c.tws, dir, 3, 256);
+    }
+  return bad;
+}
