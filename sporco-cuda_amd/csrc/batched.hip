@@ -136,8 +136,9 @@ __global__ __launch_bounds__(256) void k_brow_fwd(BList L, int sel) {
     auto body = [&](long long nrows, int mode, const float* src, float* Y,
                     float* U, float2* F, float rsf) {
       if constexpr (WAVE)
-        row_fwd_body_wave(g, nrows, rp, mode, src, Y, U, F, im.twC, rsf,
-                          im.nr, blk, lds);
+        row_fwd_body_wave(g, nrows, rp, mode, src, Y, U, F,
+                          row_wave_table(g, lds, im.twC), rsf, im.nr, blk,
+                          lds);
       else
         row_fwd_body(g, nrows, rp, mode, src, Y, U, F, im.twC, rsf, im.nr,
                      blk, lds);
@@ -258,7 +259,7 @@ __global__ __launch_bounds__(256) void k_brow(BList L, Params p,
     q.lambda = im.lambda;
     if constexpr (WAVE)
       row_body_wave(g, rp, b, q, mode, im.nr, item - L.pre[i], im.nrow, lds,
-                    red);
+                    row_wave_table(g, lds, im.twC), red);
     else
       row_body(g, rp, b, q, mode, im.nr, item - L.pre[i], im.nrow, lds, red);
     __syncthreads();

@@ -7,10 +7,11 @@
 // stage twiddles W_L^{r k L / (Ns Rx)}, takes their length-Rx DFT and writes
 // output q to (j - k) Rx + k + q Ns.
 //
-// Butterflies: radix 2, 4, 8, 16 as split-radix register DFTs with constant
-// twiddles; odd primes 3 .. 13 as register DFTs with constant coefficients
-// (inputs paired as x_r +- x_{P-r}); larger primes as a direct DFT whose work
-// items compute kQB outputs of one butterfly with recurrent phases.
+// Butterflies: radix 2, 4, 8 as split-radix register DFTs with constant
+// twiddles; odd primes 3 .. 13 as register DFTs with constant coefficients;
+// larger primes as a direct DFT whose work items compute kQB output pairs
+// of one butterfly with recurrent phases (all odd primes pair the inputs
+// as x_r +- x_{P-r}).
 //
 // The per-stage functions are __host__ __device__ and loop over work items
 // tid, tid + nthr, ..., so that the CPU check runs them for every item in
@@ -175,7 +176,7 @@ __host__ __device__ __forceinline__ GStage make_stage(int L, int Rx, int Ns) {
   return s;
 }
 
-// Register-butterfly stage (RX <= 16).
+// Register-butterfly stage (RX <= 13).
 template <int RX, int DIR>
 __host__ __device__ __forceinline__ void stage_reg(const float2* A, float2* B,
                                                    int ld, int nt,
@@ -210,10 +211,16 @@ __host__ __device__ __forceinline__ void stage_reg(const float2* A, float2* B,
   }
 }
 
-// Prime radix >= 17: a work item computes kQB outputs q0 .. q0 + kQB - 1 of
-// butterfly j.  The stage twiddle W_L^{r k tstep} and the DFT phases
-// W_Rx^{q r} advance by one complex product per input and are re-read from
-// the table every 16 inputs, which bounds their rounding drift.
+// Prime radix P >= 17, outputs paired as in dft_odd: with the twiddled
+// inputs x_r, a_r = x_r + x_{P-r}, b_r = x_r - x_{P-r} (r = 1 .. h,
+// h = (P - 1) / 2) and z = W_P^{q r} (conjugated for the inverse),
+//   t_q = x_0 + sum_r Re(z) a_r,  v_q = sum_r Im(z) b_r,
+//   X_q = t_q + i v_q,  X_{P-q} = t_q - i v_q,  X_0 = x_0 + sum_r a_r.
+// A work item computes the pairs q0 .. q0 + kQB - 1 of butterfly j (the
+// item with q0 = 1 also X_0).  The stage twiddles W_L^{r e1} and the phases
+// advance by one complex product per input pair and are re-read from the
+// table every 16 pairs, which bounds their rounding drift; the twiddle of
+// x_{P-r} is W_L^{P e1} conj(W_L^{r e1}).
 template <int DIR>
 __host__ __device__ __forceinline__ void stage_prime(const float2* A,
                                                      float2* B, int ld, int nt,
@@ -222,7 +229,8 @@ __host__ __device__ __forceinline__ void stage_prime(const float2* A,
                                                      int tid, int nthr) {
   constexpr int kQB = 4;
   const int Rx = s.Rx, nb = s.nb, Ns = s.Ns;
-  const int nq = (Rx + kQB - 1) / kQB;
+  const int h = (Rx - 1) / 2;
+  const int nq = (h + kQB - 1) / kQB;
   const int per = nb * nq;
   const GDiv dper(per), dnb(nb), dns(Ns);
   const int total = nt * per;
@@ -236,31 +244,47 @@ __host__ __device__ __forceinline__ void stage_prime(const float2* A,
     const int qb = dnb.div(rem);
     const int j = rem - qb * nb;
     const int k = s.p2 ? (j & (Ns - 1)) : j - dns.div(j) * Ns;
-    const int q0 = qb * kQB;
+    const int q0 = 1 + qb * kQB;
     const float2* in = A + t * ld;
-    const int e1 = k * s.tstep;  // stage twiddle exponent step (< L / Rx)
+    const int e1 = k * s.tstep;  // stage twiddle exponent step; Rx e1 < L
     const float2 ws = tw_at<DIR>(tw, e1, tws);
-    float2 wr = make_float2(1.f, 0.f);
-    float2 acc[kQB], z[kQB], stp[kQB];
+    const float2 wpe = tw_at<DIR>(tw, Rx * e1, tws);
+    float2 wr = ws;
+    const float2 x0 = in[gpad(j)];
+    float2 y0 = x0;
+    float2 tq[kQB], vq[kQB], z[kQB], stp[kQB];
     int qq[kQB];
 #pragma unroll
     for (int q = 0; q < kQB; ++q) {
-      qq[q] = (q0 + q < Rx) ? q0 + q : Rx - 1;
-      acc[q] = make_float2(0.f, 0.f);
-      z[q] = make_float2(1.f, 0.f);
+      qq[q] = (q0 + q <= h) ? q0 + q : h;
+      tq[q] = x0;
+      vq[q] = make_float2(0.f, 0.f);
       stp[q] = tw_at<DIR>(tw, qq[q] * nb, tws);
+      z[q] = stp[q];
     }
-    for (int r = 0; r < Rx; ++r) {
-      float2 x = in[gpad(j + r * nb)];
-      if (k) x = c_mul(x, wr);
+#pragma unroll 1
+    for (int r = 1; r <= h; ++r) {
+      float2 xa = in[gpad(j + r * nb)];
+      float2 xb = in[gpad(j + (Rx - r) * nb)];
+      if (k) {
+        xa = c_mul(xa, wr);
+        xb = c_mul(xb, c_mul(wpe, make_float2(wr.x, -wr.y)));
+      }
+      const float2 a = c_add(xa, xb);
+      const float2 b = c_sub(xa, xb);
+      y0 = c_add(y0, a);
 #pragma unroll
-      for (int q = 0; q < kQB; ++q) acc[q] = c_add(acc[q], c_mul(x, z[q]));
+      for (int q = 0; q < kQB; ++q) {
+        tq[q].x += z[q].x * a.x;
+        tq[q].y += z[q].x * a.y;
+        vq[q].x += z[q].y * b.x;
+        vq[q].y += z[q].y * b.y;
+      }
       if (((r + 1) & 15) == 0) {
-        if (r + 1 < Rx) wr = tw_at<DIR>(tw, e1 * (r + 1), tws);
+        wr = tw_at<DIR>(tw, e1 * (r + 1), tws);
 #pragma unroll
-        for (int q = 0; q < kQB; ++q) {
+        for (int q = 0; q < kQB; ++q)
           z[q] = tw_at<DIR>(tw, ((qq[q] * (r + 1)) % Rx) * nb, tws);
-        }
       } else {
         wr = c_mul(wr, ws);
 #pragma unroll
@@ -269,9 +293,14 @@ __host__ __device__ __forceinline__ void stage_prime(const float2* A,
     }
     float2* out = B + t * ld;
     const int ob = (j - k) * Rx + k;
+    if (qb == 0) out[gpad(ob)] = y0;
 #pragma unroll
     for (int q = 0; q < kQB; ++q)
-      if (q0 + q < Rx) out[gpad(ob + (q0 + q) * Ns)] = acc[q];
+      if (q0 + q <= h) {
+        const float2 iv = make_float2(-vq[q].y, vq[q].x);
+        out[gpad(ob + (q0 + q) * Ns)] = c_add(tq[q], iv);
+        out[gpad(ob + (Rx - q0 - q) * Ns)] = c_sub(tq[q], iv);
+      }
   }
 }
 

@@ -330,15 +330,26 @@ __device__ inline void row_body(const Geom& g, const FftPlan& rp,
 // Wavefront-per-row bodies: the block's kRowWaves wavefronts take rows
 // blk*nr + wv, + kRowWaves, ... (nr rows per block), each in its own pair of
 // LDS buffers of gld(Lr) complex values, ordered by wavefront sync only.
-// Used when the two buffers of 4 wavefronts fit 64 KiB (gld(Lr) <= 1024).
+// The row twiddle table (C values W_C^t) follows the buffers in LDS.  Used
+// when buffers and table fit 64 KiB.
 // --------------------------------------------------------------------------
 constexpr int kRowWaves = 4;
 
-__host__ __device__ inline bool row_wave_fits(const Geom& g) {
-  return gen::gld(g.Lr) <= 1024;
-}
 __host__ __device__ inline int row_wave_lds(const Geom& g) {
-  return kRowWaves * 2 * gen::gld(g.Lr) * 8;
+  return (kRowWaves * 2 * gen::gld(g.Lr) + g.C) * 8;
+}
+__host__ __device__ inline bool row_wave_fits(const Geom& g) {
+  return row_wave_lds(g) <= 65536;
+}
+
+// Copy of the row twiddle table into the LDS after the wavefront buffers;
+// every thread of the block calls it (ends with a block barrier).
+__device__ inline float2* row_wave_table(const Geom& g, float2* lds,
+                                         const float2* __restrict__ twC) {
+  float2* T = lds + kRowWaves * 2 * gen::gld(g.Lr);
+  for (int i = threadIdx.x; i < g.C; i += blockDim.x) T[i] = twC[i];
+  __syncthreads();
+  return T;
 }
 
 // Offset of (r, c) in one map's panel layout; TC = 1 << tsh.
@@ -410,12 +421,12 @@ __device__ __forceinline__ void wave_r2c_store(const Geom& g, int tsh,
 // Row kernel body, one wavefront per row: inverse row FFT -> X, fused
 // update and residual partial sums (reduced over the block into
 // partials[j * nparts + blk]), then in fused mode the forward row FFT of
-// Y - U.
+// Y - U.  twL: the row twiddle table in LDS (row_wave_table).
 __device__ inline void row_body_wave(const Geom& g, const FftPlan& rp,
                                      const Buffers& b, const Params& p,
                                      int mode, int nr, long long blk,
                                      int nparts, float2* lds,
-                                     float (*red)[4]) {
+                                     const float2* twL, float (*red)[4]) {
   const int ld = gen::gld(g.Lr);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   float2* A = lds + wv * 2 * ld;
@@ -433,8 +444,8 @@ __device__ inline void row_body_wave(const Geom& g, const FftPlan& rp,
     const int m = (int)(row / g.R);
     const int r = (int)(row - (long long)m * g.R);
     float2* Fm = b.F + (long long)m * g.Hc;
-    wave_c2r_load(g, tsh, Fm, r, A, B, b.twC, lane);
-    float2* P = fft_wave<-1>(A, B, rp, b.twC, lane);
+    wave_c2r_load(g, tsh, Fm, r, A, B, twL, lane);
+    float2* P = fft_wave<-1>(A, B, rp, twL, lane);
     float2* Q = (P == A) ? B : A;
     const long long gi0 = row * C;
     for (int n = lane; n < C; n += 64) {
@@ -450,8 +461,8 @@ __device__ inline void row_body_wave(const Geom& g, const FftPlan& rp,
     }
     if (mode == kRowFused) {
       reg::wave_sync();
-      const float2* Z = fft_wave<1>(P, Q, rp, b.twC, lane);
-      wave_r2c_store(g, tsh, Z, Fm, r, b.twC, lane);
+      const float2* Z = fft_wave<1>(P, Q, rp, twL, lane);
+      wave_r2c_store(g, tsh, Z, Fm, r, twL, lane);
     }
     reg::wave_sync();  // this row's LDS reads before the next row's writes
   }
@@ -459,12 +470,12 @@ __device__ inline void row_body_wave(const Geom& g, const FftPlan& rp,
 }
 
 // Forward row transform, one wavefront per row, of rows blk*nr.. of a real
-// array (setup) or of Y - U/rsf (AutoRho), into F.
+// array (setup) or of Y - U/rsf (AutoRho), into F.  twL: the row twiddle
+// table in LDS.
 __device__ inline void row_fwd_body_wave(const Geom& g, long long nrows,
                                          const FftPlan& rp, int mode,
                                          const float* src, float* Y, float* U,
-                                         float2* F,
-                                         const float2* __restrict__ twC,
+                                         float2* F, const float2* twL,
                                          float rsf, int nr, long long blk,
                                          float2* lds) {
   const int ld = gen::gld(g.Lr);
@@ -498,8 +509,8 @@ __device__ inline void row_fwd_body_wave(const Geom& g, long long nrows,
         A[gen::gpad(n)] = make_float2(a, 0.f);
     }
     reg::wave_sync();
-    const float2* Z = fft_wave<1>(A, B, rp, twC, lane);
-    wave_r2c_store(g, tsh, Z, F + (long long)m * g.Hc, r, twC, lane);
+    const float2* Z = fft_wave<1>(A, B, rp, twL, lane);
+    wave_r2c_store(g, tsh, Z, F + (long long)m * g.Hc, r, twL, lane);
     reg::wave_sync();
   }
 }

@@ -60,8 +60,8 @@ __global__ __launch_bounds__(256) void k_row_fwd_generic(
   extern __shared__ float2 lds[];
   const float rsf = (mode == kFwdYU) ? ctrl->rsf : 1.f;
   if constexpr (WAVE)
-    row_fwd_body_wave(g, nrows, rp, mode, src, Y, U, F, twC, rsf, nr,
-                      blockIdx.x, lds);
+    row_fwd_body_wave(g, nrows, rp, mode, src, Y, U, F,
+                      row_wave_table(g, lds, twC), rsf, nr, blockIdx.x, lds);
   else
     row_fwd_body(g, nrows, rp, mode, src, Y, U, F, twC, rsf, nr, blockIdx.x,
                  lds);
@@ -145,7 +145,8 @@ __global__ __launch_bounds__(256) void k_row_generic(Geom g, FftPlan rp,
   extern __shared__ float2 lds[];
   __shared__ float red[5][4];
   if constexpr (WAVE)
-    row_body_wave(g, rp, b, p, mode, nr, blockIdx.x, gridDim.x, lds, red);
+    row_body_wave(g, rp, b, p, mode, nr, blockIdx.x, gridDim.x, lds,
+                  row_wave_table(g, lds, b.twC), red);
   else
     row_body(g, rp, b, p, mode, nr, blockIdx.x, gridDim.x, lds, red);
 }

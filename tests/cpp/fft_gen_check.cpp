@@ -105,7 +105,7 @@ int main() {
   const std::vector<Case> cases = {
       {384, {8, 8, 2, 3}, 2},   {384, {8, 4, 4, 3}, 2}, {356, {4, 89}, 2},
       {64, {8, 8}, 1},          {96, {8, 4, 3}, 1},     {128, {8, 4, 4}, 1},
-      {2, {2}, 1},              {4, {4}, 2},            {48, {16, 3}, 1},
+      {2, {2}, 1},              {4, {4}, 2},            {48, {4, 4, 3}, 1},
       {45, {3, 3, 5}, 1},       {105, {3, 5, 7}, 1},    {143, {11, 13}, 1},
       {264, {8, 3, 11}, 2},     {17, {17}, 1},          {31, {31}, 1},
       {232, {8, 29}, 2},        {712, {8, 89}, 1},      {664, {8, 83}, 2},
