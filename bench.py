@@ -193,7 +193,8 @@ def batched_leg(seed=3, n_images=64, iters=150):
         'loop_Mpix_iters_per_s': N * iters / tl / 1e6,
         'max_rel_residual_diff': worst, 'max_rel_Y_diff': ydiff,
         'config': {k: cfg[k] for k in ('col_lds', 'row_lds', 'solve_lanes',
-                                       'solve_maps', 'persistent', 'graph')},
+                                       'solve_maps', 'persistent', 'graph',
+                                       'branches')},
         'device_solve_ms': cfg['solve_ms'], 'device_setup_ms': cfg['setup_ms'],
         'first_call_tune_ms': cfg0['tune_ms'],
     }

@@ -194,11 +194,13 @@ int scc_reconstruct_backward(const float *D, int M, int Kh, int Kw,
 
 /* Batched plain CBPDN: nimg images of any sizes sharing one dictionary D
  * (M, Kh, Kw), each with its own lambda, solved together.  Every ADMM stage
- * is one launch over a work list that spans all images, and the iteration
- * loop is replayed from a captured HIP graph; panel width, rows per row
- * block, lanes per frequency of the solve and the grid shape are chosen by
- * timing candidates on the batch itself the first time a batch signature
- * (device, M, image sizes) is seen.  Per image the result equals scc_cbpdn
+ * is one launch over a work list that spans all images, or every image runs
+ * the single-image kernels of its size as a chain of launches, the chains
+ * spread over up to 4 branches; either way the iteration loop is replayed
+ * from a captured HIP graph.  The mode, panel width, rows per row block,
+ * lanes per frequency of the solve and the grid shape are chosen by timing
+ * candidates on the batch itself the first time a batch signature (device,
+ * M, image sizes) is seen.  Per image the result equals scc_cbpdn
  * on that image (same iteration, residuals, stopping test and AutoRho,
  * applied per image); options are read from opt as there, except the size
  * fields, which come from the arguments.  L1Weight may hold 1 or M entries;
@@ -230,6 +232,8 @@ typedef struct {
   double tune_ms;    /* autotuning time of this call (0 when cached) */
   double setup_ms;   /* spectra of D and the images (device time) */
   double solve_ms;   /* iterations (device time) */
+  int branches;      /* 0: work-list kernels; k > 0: per-image kernel
+                        chains on k graph branches */
 } scc_batch_config;
 
 int scc_batched_cbpdn(const float *D, int M, int Kh, int Kw, int nimg,
@@ -238,6 +242,9 @@ int scc_batched_cbpdn(const float *D, int M, int Kh, int Kw, int nimg,
 /* Tuning policy of later calls: 1 autotune new signatures (default), 0 use
  * the fixed default configuration. */
 int scc_batched_set_autotune(int on);
+/* Mode of later calls: k < 0 the tuned mode (default), 0 work-list
+ * kernels, k > 0 per-image kernel chains on min(k, 4) graph branches. */
+int scc_batched_set_branches(int k);
 /* Graph policy: 1 capture the iterations in a HIP graph (default), 0 launch
  * them directly. */
 int scc_batched_set_graph(int on);

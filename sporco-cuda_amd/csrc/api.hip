@@ -96,11 +96,7 @@ int make_geom(int R, int C, int M, Geom* g, bool* pow2, Pow2Config* p2) {
   if (*pow2) {
     TC = p2->TC;
   } else {
-    int cap = 65536 / (2 * gen::gld(R) * 8);
-    if (cap < 1) cap = 1;
-    cap = std::min(cap, g->H);
-    TC = 1;
-    while (TC * 2 <= cap) TC *= 2;
+    TC = gen::col_panel_width(R, g->H, 65536);
   }
   g->TC = TC;
   g->P = (g->H + TC - 1) / TC;

@@ -28,6 +28,7 @@
 
 #include "../../include/sporco_amd.h"
 #include "generic_common.h"
+#include "scc_pow2.h"
 
 namespace scc {
 
@@ -129,27 +130,29 @@ __global__ __launch_bounds__(256) void k_brow_fwd(BList L, int sel) {
     // to be reloaded inside the transform loops.  The plan is read once per
     // stage and stays in memory (a local copy indexed by stage would live in
     // scratch).
-    const Geom g = im.g;
+    const Geom g = geom_uni(im.g);
     const FftPlan& rp = im.rp;
-    const long long blk = item - L.pre[i];
+    const long long blk = uni(item - L.pre[i]);
     const long long mr = (long long)g.M * g.R;
+    const int nr = uni(im.nr);
+    const float2* twC = uni(im.twC);
     auto body = [&](long long nrows, int mode, const float* src, float* Y,
                     float* U, float2* F, float rsf) {
       if constexpr (WAVE)
         row_fwd_body_wave(g, nrows, rp, mode, src, Y, U, F,
-                          row_wave_table(g, lds, im.twC), rsf, im.nr, blk,
-                          lds);
+                          row_wave_table(g, lds, twC), rsf, nr, blk, lds);
       else
-        row_fwd_body(g, nrows, rp, mode, src, Y, U, F, im.twC, rsf, im.nr,
-                     blk, lds);
+        row_fwd_body(g, nrows, rp, mode, src, Y, U, F, twC, rsf, nr, blk,
+                     lds);
     };
     if (sel == kSelWork) {
       if (im.ctrl->done) continue;
-      body(mr, kFwdYU, nullptr, im.Y, im.U, im.F, im.ctrl->rsf);
+      body(mr, kFwdYU, nullptr, uni(im.Y), uni(im.U), uni(im.F),
+           uni(im.ctrl->rsf));
     } else if (sel == kSelDict) {
-      body(mr, kFwdReal, im.Y, nullptr, nullptr, im.Df, 1.f);
+      body(mr, kFwdReal, uni(im.Y), nullptr, nullptr, uni(im.Df), 1.f);
     } else {
-      body(g.R, kFwdReal, im.S, nullptr, nullptr, im.Sf, 1.f);
+      body(g.R, kFwdReal, uni(im.S), nullptr, nullptr, uni(im.Sf), 1.f);
     }
     __syncthreads();  // LDS is reused by the next item
   }
@@ -162,13 +165,15 @@ __global__ __launch_bounds__(256) void k_bcol(BList L, int sel, float dir) {
     const int i = img_of(L, item);
     const BImg& im = L.im[i];
     if (sel == kSelWork && im.ctrl->done) continue;
-    const Geom g = im.g;
+    const Geom g = geom_uni(im.g);
     const FftPlan& cp = im.cp;
-    const int local = item - L.pre[i];
+    const int local = uni(item - L.pre[i]);
     const int m = local / g.P;
     const int p = local - m * g.P;
-    float2* F = sel == kSelDict ? im.Df : (sel == kSelImage ? im.Sf : im.F);
-    col_body(g, cp, F, im.twR, dir, p, m, lds);
+    float2* F =
+        uni(sel == kSelDict ? im.Df : (sel == kSelImage ? im.Sf : im.F));
+    const float2* twR = uni(im.twR);
+    col_body(g, cp, F, twR, dir, p, m, lds);
     __syncthreads();
   }
 }
@@ -245,7 +250,7 @@ __global__ __launch_bounds__(256) void k_brow(BList L, Params p,
     const int i = img_of(L, item);
     const BImg& im = L.im[i];
     if (im.ctrl->done) continue;
-    const Geom g = im.g;
+    const Geom g = geom_uni(im.g);
     const FftPlan& rp = im.rp;
     Buffers b{};
     b.Y = im.Y;
@@ -342,6 +347,21 @@ struct BCfg {
   int rpw = 2;          // rows per wavefront, wavefront-per-row kernels
   int mpt = 8;
   int persistent = 1;
+  int branches = 0;     // 0: work-list kernels; k > 0: per-image chains
+};
+
+constexpr int kMaxBranches = 4;
+
+// Per-image chains mode: every image runs the single-image kernels of its
+// own size (power-of-two or generic, with the single-image geometry and
+// panel layout) in a chain of launches; the chains of all images are
+// captured on `branches` streams of one graph, so that the images of
+// different branches overlap on the device.
+struct BChain {
+  Geom g;
+  Pow2Config p2;
+  bool pow2;
+  int nblk;  // row-kernel blocks = residual partial sums
 };
 
 // Work lists: 0 row items of M maps, 1 row items of the image, 2 column
@@ -354,6 +374,7 @@ struct BWs {
   std::vector<int> sizes;  // R0, C0, R1, C1, ...
   int nimg = 0;
   BCfg cfg;
+  BCfg tcfg;  // tuned (or default) configuration of this signature
   bool tuned = false;
   std::vector<BImg> h;
   std::vector<int> hpre;  // kNumLists x (nimg + 1)
@@ -362,7 +383,14 @@ struct BWs {
   bool wave = false;  // wavefront-per-row kernels (every row fits)
   int grid[kNumLists] = {};
   int G = 1;
-  BBuf U, F, Df, Sf, part, ctrl, tw, desc, pre, l1w;
+  BBuf U, F, Df, Sf, part, ctrl, tw, desc, pre, l1w, bsum;
+  // Chains mode: the iterations run on Yw (same offsets as U), copied to
+  // the caller's outputs afterwards, so that a captured graph holds no
+  // per-call pointer.
+  BBuf Yw;
+  std::vector<BChain> ch;
+  hipStream_t br[kMaxBranches] = {};
+  hipEvent_t bev[kMaxBranches + 1] = {};
   std::vector<size_t> tw_off;
   hipStream_t cap = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -373,6 +401,10 @@ struct BWs {
   ~BWs() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (cap) (void)hipStreamDestroy(cap);
+    for (auto& s : br)
+      if (s) (void)hipStreamDestroy(s);
+    for (auto& e : bev)
+      if (e) (void)hipEventDestroy(e);
     if (hctrl) (void)hipHostFree(hctrl);
     if (hdesc) (void)hipHostFree(hdesc);
     for (auto& e : ev)
@@ -384,6 +416,7 @@ std::mutex b_mu;
 std::unique_ptr<BWs> b_ws;
 bool b_autotune = true;
 bool b_graph = true;
+int b_branches = -1;  // < 0: tuned mode; 0: work lists; k > 0: k chains
 thread_local scc_batch_config b_last{};
 
 BList list_of(const BWs* w, int k) {
@@ -419,10 +452,7 @@ int configure(BWs* w, const BCfg& c, hipStream_t st) {
     BImg& im = w->h[i];
     Geom& g = im.g;
     const int R = g.R;
-    int cap = c.col_lds / (2 * gen::gld(R) * 8);
-    cap = std::max(1, std::min(cap, g.H));
-    int TC = 1;
-    while (TC * 2 <= cap) TC *= 2;
+    const int TC = gen::col_panel_width(R, g.H, c.col_lds);
     g.TC = TC;
     g.P = (g.H + TC - 1) / TC;
     g.wlast = g.H - (g.P - 1) * TC;
@@ -525,8 +555,19 @@ int get_bws(int device, int M, const std::vector<int>& sizes, BWs** out) {
       (rc = w->ctrl.get((size_t)n * sizeof(Ctrl))) ||
       (rc = w->tw.get(ntw * 8)) || (rc = w->desc.get((size_t)n * sizeof(BImg))) ||
       (rc = w->pre.get((size_t)kNumLists * (n + 1) * sizeof(int))) ||
-      (rc = w->l1w.get((size_t)M * 4)))
+      (rc = w->l1w.get((size_t)M * 4)) ||
+      (rc = w->bsum.get((size_t)n * 5 * 64 * 8)))
     return rc;
+  w->ch.resize(n);
+  for (int i = 0; i < n; ++i) {
+    BChain& c = w->ch[i];
+    make_geom(sizes[2 * i], sizes[2 * i + 1], M, &c.g, &c.pow2, &c.p2);
+    c.nblk = c.pow2 ? pow2_row_blocks(c.g, c.p2) : row_blocks(c.g);
+  }
+  for (auto& s2 : w->br)
+    BHIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+  for (auto& e : w->bev)
+    BHIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   BHIP(hipHostMalloc((void**)&w->hctrl, (size_t)n * sizeof(Ctrl)));
   BHIP(hipHostMalloc((void**)&w->hdesc, (size_t)n * sizeof(BImg)));
   BHIP(hipStreamCreateWithFlags(&w->cap, hipStreamNonBlocking));
@@ -630,6 +671,130 @@ int upload_ctrl(BWs* w, float rho, hipStream_t st) {
   return SCC_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Per-image chains (BCfg::branches > 0)
+// ---------------------------------------------------------------------------
+Buffers chain_buffers(const BWs* w, int j) {
+  const BImg& im = w->h[j];
+  Buffers b{};
+  b.Y = im.Y;
+  b.U = im.U;
+  b.F = im.F;
+  b.Df = im.Df;
+  b.Sf = im.Sf;
+  b.l1w = (const float*)w->l1w.p;
+  b.partials = im.partials;
+  b.blocksum = (double*)w->bsum.p + (size_t)j * 5 * 64;
+  b.ctrl = (Ctrl*)w->ctrl.p + j;
+  b.twR = im.twR;
+  b.twC = im.twC;
+  return b;
+}
+
+Params chain_params(const BWs* w, int j, const Params& p0) {
+  Params p = p0;
+  p.lambda = w->h[j].lambda;
+  p.xi = w->h[j].xi;
+  p.sqrt_nc = w->h[j].sqrt_nc;
+  return p;
+}
+
+// Spectra of the padded filters (held in the image's Y) and of the image.
+int chain_setup(BWs* w, int j, hipStream_t st) {
+  const BChain& c = w->ch[j];
+  const BImg& im = w->h[j];
+  const Geom& g = c.g;
+  const Buffers b = chain_buffers(w, j);
+  if (c.pow2) {
+    BHIP(pow2_row_fwd(g, c.p2, g.M, kFwdReal, b.Y, nullptr, nullptr, b.Df,
+                      b.twC, nullptr, st));
+    BHIP(pow2_col(g, c.p2, g.M, b.Df, b.twR, 0, nullptr, st));
+    BHIP(pow2_row_fwd(g, c.p2, 1, kFwdReal, im.S, nullptr, nullptr, b.Sf,
+                      b.twC, nullptr, st));
+    BHIP(pow2_col(g, c.p2, 1, b.Sf, b.twR, 0, nullptr, st));
+  } else {
+    BHIP(launch_row_fwd(g, g.M, im.rp, kFwdReal, b.Y, nullptr, nullptr, b.Df,
+                        b.twC, nullptr, st));
+    BHIP(launch_col(g, g.M, im.cp, b.Df, b.twR, 0, nullptr, st));
+    BHIP(launch_row_fwd(g, 1, im.rp, kFwdReal, im.S, nullptr, nullptr, b.Sf,
+                        b.twC, nullptr, st));
+    BHIP(launch_col(g, 1, im.cp, b.Sf, b.twR, 0, nullptr, st));
+  }
+  return SCC_OK;
+}
+
+// Iteration i of image j: the single-image sequence (solve_core of the
+// single-image solver), with the power-of-two path's deferred residual
+// reduction on every iteration that is neither an AutoRho step nor the
+// last one.
+bool chain_defers(const BChain& c, const Params& p, int i, int maxit) {
+  const bool rho_step = p.autorho && i > 1 && i % p.period == 0;
+  return c.pow2 && !rho_step && i < maxit;
+}
+
+int chain_iter(BWs* w, int j, const Params& p0, int i, int maxit,
+               hipStream_t st) {
+  const BChain& c = w->ch[j];
+  const BImg& im = w->h[j];
+  const Geom& g = c.g;
+  const Buffers b = chain_buffers(w, j);
+  const Params p = chain_params(w, j, p0);
+  const bool rho_step = p.autorho && i > 1 && i % p.period == 0;
+  const int rmode = rho_step ? kRowUpdate : kRowFused;
+  const bool pending = i > 1 && chain_defers(c, p, i - 1, maxit);
+  int nblocks = c.nblk;
+  if (c.pow2) {
+    BHIP(pow2_col(g, c.p2, g.M, b.F, b.twR, 0, b.ctrl, st,
+                  pending ? b.partials : nullptr, pending ? c.nblk : 0,
+                  b.blocksum));
+    BHIP(pow2_solve(g, c.p2, b, p, st, 1, pending ? i - 1 : 0,
+                    pending ? pow2_defer_blocks(g, g.M) : 0));
+    BHIP(pow2_col(g, c.p2, g.M, b.F, b.twR, 1, b.ctrl, st));
+    BHIP(pow2_row(g, c.p2, b, p, rmode, &nblocks, st));
+  } else {
+    BHIP(launch_col(g, g.M, im.cp, b.F, b.twR, 0, b.ctrl, st));
+    BHIP(launch_solve(g, b, p, st));
+    BHIP(launch_col(g, g.M, im.cp, b.F, b.twR, 1, b.ctrl, st));
+    BHIP(launch_row(g, im.rp, b, p, rmode, &nblocks, st));
+  }
+  if (!chain_defers(c, p, i, maxit))
+    BHIP(launch_reduce(b, p, nblocks, i, maxit, st));
+  if (rho_step) {
+    if (c.pow2)
+      BHIP(pow2_row_fwd(g, c.p2, g.M, kFwdYU, nullptr, b.Y, b.U, b.F, b.twC,
+                        b.ctrl, st));
+    else
+      BHIP(launch_row_fwd(g, g.M, im.rp, kFwdYU, nullptr, b.Y, b.U, b.F,
+                          b.twC, b.ctrl, st));
+  }
+  return SCC_OK;
+}
+
+// Iterations i0 .. i1 (of maxit) of every image: image j on branch
+// j mod branches, forked from and joined to st.
+int launch_chains(BWs* w, const Params& p, int i0, int i1, int maxit,
+                  hipStream_t st) {
+  const int K = std::min(std::max(w->cfg.branches, 1), kMaxBranches);
+  int rc;
+  if (K == 1) {
+    for (int j = 0; j < w->nimg; ++j)
+      for (int i = i0; i <= i1; ++i)
+        if ((rc = chain_iter(w, j, p, i, maxit, st))) return rc;
+    return SCC_OK;
+  }
+  BHIP(hipEventRecord(w->bev[kMaxBranches], st));
+  for (int k = 0; k < K; ++k)
+    BHIP(hipStreamWaitEvent(w->br[k], w->bev[kMaxBranches], 0));
+  for (int j = 0; j < w->nimg; ++j)
+    for (int i = i0; i <= i1; ++i)
+      if ((rc = chain_iter(w, j, p, i, maxit, w->br[j % K]))) return rc;
+  for (int k = 0; k < K; ++k) {
+    BHIP(hipEventRecord(w->bev[k], w->br[k]));
+    BHIP(hipStreamWaitEvent(st, w->bev[k], 0));
+  }
+  return SCC_OK;
+}
+
 // Average device time of `reps` runs of fn after one warm-up run.
 template <class Fn>
 int time_it(BWs* w, hipStream_t st, int reps, Fn fn, float* ms) {
@@ -718,6 +883,33 @@ int autotune(BWs* w, const Params& p0, hipStream_t st) {
       best.persistent = pers;
     }
   }
+  // Work lists against per-image chains on 1, 2 or 4 branches, over two
+  // iterations (with the power-of-two images' deferred reduction).
+  if ((rc = configure(w, best, st)) || (rc = reset())) return rc;
+  if ((rc = time_it(w, st, 2,
+                    [&] {
+                      launch_iter(w, p, 1, st);
+                      launch_iter(w, p, 2, st);
+                    },
+                    &bt)))
+    return rc;
+  for (int k : {1, 2, 4}) {
+    BCfg c = best;
+    c.branches = k;
+    if ((rc = configure(w, c, st)) || (rc = reset())) return rc;
+    int lrc = SCC_OK;
+    if ((rc = time_it(w, st, 2,
+                      [&] {
+                        if (lrc == SCC_OK) lrc = launch_chains(w, p, 1, 2, 2, st);
+                      },
+                      &t)))
+      return rc;
+    if (lrc) return lrc;
+    if (t < bt) {
+      bt = t;
+      best.branches = k;
+    }
+  }
   return configure(w, best, st);
 }
 
@@ -737,6 +929,12 @@ int scc_batched_set_autotune(int on) {
 int scc_batched_set_graph(int on) {
   std::lock_guard<std::mutex> lk(b_mu);
   b_graph = on != 0;
+  return SCC_OK;
+}
+
+int scc_batched_set_branches(int k) {
+  std::lock_guard<std::mutex> lk(b_mu);
+  b_branches = k < 0 ? -1 : k;
   return SCC_OK;
 }
 
@@ -870,21 +1068,39 @@ int scc_batched_cbpdn(const float* D, int M, int Kh, int Kw, int nimg,
       b_last.tune_ms = ms;
     }
     w->tuned = true;
+    w->tcfg = w->cfg;
     if (w->exec) {
       (void)hipGraphExecDestroy(w->exec);
       w->exec = nullptr;
     }
   }
-  if ((rc = configure(w, w->cfg, st))) return rc;
+  BCfg use = w->tcfg;
+  if (b_branches >= 0) use.branches = std::min(b_branches, kMaxBranches);
+  const bool chains = use.branches > 0;
+  if (chains) {
+    size_t nreal = 0;
+    for (int i = 0; i < nimg; ++i)
+      nreal = std::max(nreal, (size_t)(w->h[i].U - (float*)w->U.p) +
+                                  (size_t)M * (size_t)w->h[i].g.N);
+    if ((rc = w->Yw.get(nreal * 4))) return rc;
+    for (int i = 0; i < nimg; ++i)
+      w->h[i].Y = (float*)w->Yw.p + (w->h[i].U - (float*)w->U.p);
+  }
+  if ((rc = configure(w, use, st))) return rc;
 
   // Setup: spectra of the padded filters (per image size) and the images.
   BHIP(hipEventRecord(w->ev[0], st));
   hipLaunchKernelGGL(k_bpad, dim3(512, nimg), dim3(256), 0, st,
                      (const BImg*)w->desc.p, D, Kh, Kw);
-  launch_row_fwd_b(w, kLRowM, kSelDict, st);
-  launch_col_b(w, kLColM, kSelDict, 1.f, st);
-  launch_row_fwd_b(w, kLRow1, kSelImage, st);
-  launch_col_b(w, kLCol1, kSelImage, 1.f, st);
+  if (chains) {
+    for (int j = 0; j < nimg; ++j)
+      if ((rc = chain_setup(w, j, st))) return rc;
+  } else {
+    launch_row_fwd_b(w, kLRowM, kSelDict, st);
+    launch_col_b(w, kLColM, kSelDict, 1.f, st);
+    launch_row_fwd_b(w, kLRow1, kSelImage, st);
+    launch_col_b(w, kLCol1, kSelImage, 1.f, st);
+  }
   launch_zero(w, st);
   if ((rc = upload_ctrl(w, 0.f, st))) return rc;
   BHIP(hipGetLastError());
@@ -893,11 +1109,19 @@ int scc_batched_cbpdn(const float* D, int M, int Kh, int Kw, int nimg,
   const int maxit = o.MaxMainIter;
   bool graphed = false;
   if (b_graph && maxit > 0) {
-    std::vector<unsigned char> key(sizeof(Params) + 2 * sizeof(int));
+    std::vector<unsigned char> key(sizeof(Params) + 3 * sizeof(int));
     std::memcpy(key.data(), &p, sizeof(Params));
     std::memcpy(key.data() + sizeof(Params), &maxit, sizeof(int));
     const int nl = (int)nl1;
     std::memcpy(key.data() + sizeof(Params) + sizeof(int), &nl, sizeof(int));
+    std::memcpy(key.data() + sizeof(Params) + 2 * sizeof(int), &use.branches,
+                sizeof(int));
+    if (chains)  // the chains' kernels take lambda and xi by value
+      for (int i = 0; i < nimg; ++i) {
+        const float v[2] = {w->h[i].lambda, w->h[i].xi};
+        const unsigned char* b8 = (const unsigned char*)v;
+        key.insert(key.end(), b8, b8 + sizeof(v));
+      }
     if (w->exec == nullptr || key != w->gkey) {
       if (w->exec) {
         (void)hipGraphExecDestroy(w->exec);
@@ -905,8 +1129,11 @@ int scc_batched_cbpdn(const float* D, int M, int Kh, int Kw, int nimg,
       }
       BHIP(hipStreamBeginCapture(w->cap, hipStreamCaptureModeThreadLocal));
       int lrc = SCC_OK;
-      for (int i = 1; i <= maxit && lrc == SCC_OK; ++i)
-        lrc = launch_iter(w, p, i, w->cap);
+      if (chains)
+        lrc = launch_chains(w, p, 1, maxit, maxit, w->cap);
+      else
+        for (int i = 1; i <= maxit && lrc == SCC_OK; ++i)
+          lrc = launch_iter(w, p, i, w->cap);
       hipGraph_t graph = nullptr;
       const hipError_t ec = hipStreamEndCapture(w->cap, &graph);
       if (lrc) {
@@ -922,11 +1149,19 @@ int scc_batched_cbpdn(const float* D, int M, int Kh, int Kw, int nimg,
     }
     BHIP(hipGraphLaunch(w->exec, st));
     graphed = true;
+  } else if (chains) {
+    if (maxit > 0 && (rc = launch_chains(w, p, 1, maxit, maxit, st)))
+      return rc;
   } else {
     for (int i = 1; i <= maxit; ++i)
       if ((rc = launch_iter(w, p, i, st))) return rc;
   }
   BHIP(hipEventRecord(w->ev[2], st));
+  if (chains)
+    for (int i = 0; i < nimg; ++i)
+      BHIP(hipMemcpyAsync(images[i].Y, w->h[i].Y,
+                          (size_t)M * w->h[i].g.N * 4,
+                          hipMemcpyDeviceToDevice, st));
   BHIP(hipMemcpyAsync(w->hctrl, w->ctrl.p, (size_t)nimg * sizeof(Ctrl),
                       hipMemcpyDeviceToHost, st));
   BHIP(hipStreamSynchronize(st));
@@ -946,6 +1181,7 @@ int scc_batched_cbpdn(const float* D, int M, int Kh, int Kw, int nimg,
   b_last.solve_lanes = w->G;
   b_last.solve_maps = w->cfg.mpt;
   b_last.persistent = w->cfg.persistent;
+  b_last.branches = w->cfg.branches;
   b_last.graph = graphed ? 1 : 0;
   if (hipEventElapsedTime(&ms, w->ev[0], w->ev[1]) == hipSuccess)
     b_last.setup_ms = ms;

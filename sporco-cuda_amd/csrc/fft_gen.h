@@ -47,6 +47,17 @@ __host__ __device__ inline bool sides_fit_lds(int R, int C) {
   return R >= 1 && C >= 1 && 16 * gld(R) <= 65536 && 16 * gld(Lr) <= 61440;
 }
 
+// Widest power-of-two panel (<= H columns) whose block-wide column
+// transform (two padded buffers per column) fits `budget` bytes of LDS (at
+// least one column).
+__host__ __device__ inline int col_panel_width(int R, int H, int budget) {
+  int cap = budget / (16 * gld(R));
+  if (cap > H) cap = H;
+  int TC = 1;
+  while (TC * 2 <= cap) TC *= 2;
+  return TC;
+}
+
 // Quotient by a divisor fixed over a loop, for operands below 2^22: the
 // float estimate is within one of the quotient, corrected once each way.
 struct GDiv {

@@ -43,19 +43,28 @@ struct FastDiv {
 // thread of the block calls it (block barriers between stages); fft_wave:
 // one transform per wavefront (nt = 1), wavefront ordering only.
 // --------------------------------------------------------------------------
+// The plan is read through readfirstlane: it may live in global memory (the
+// batched descriptors), where the compiler cannot see that it is uniform,
+// and a divergent radix would turn the stage dispatch into divergent code.
+__device__ __forceinline__ int uni(int v) {
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
 template <int DIR>
 __device__ float2* fft_block(float2* A, float2* B, int nt, int ld,
                              const FftPlan& pl, const float2* __restrict__ tw,
                              int tid, int nthr) {
+  const int nst = uni(pl.nstages), L = uni(pl.L), tws = uni(pl.tw_stride);
   int Ns = 1;
-  for (int s = 0; s < pl.nstages; ++s) {
-    const gen::GStage st = gen::make_stage(pl.L, pl.radix[s], Ns);
-    gen::stage<DIR>(A, B, ld, nt, st, tw, pl.tw_stride, tid, nthr);
+  for (int s = 0; s < nst; ++s) {
+    const int Rx = uni(pl.radix[s]);
+    const gen::GStage st = gen::make_stage(L, Rx, Ns);
+    gen::stage<DIR>(A, B, ld, nt, st, tw, tws, tid, nthr);
     __syncthreads();
     float2* tmp = A;
     A = B;
     B = tmp;
-    Ns *= pl.radix[s];
+    Ns *= Rx;
   }
   return A;
 }
@@ -63,17 +72,48 @@ __device__ float2* fft_block(float2* A, float2* B, int nt, int ld,
 template <int DIR>
 __device__ float2* fft_wave(float2* A, float2* B, const FftPlan& pl,
                             const float2* __restrict__ tw, int lane) {
+  const int nst = uni(pl.nstages), L = uni(pl.L), tws = uni(pl.tw_stride);
   int Ns = 1;
-  for (int s = 0; s < pl.nstages; ++s) {
-    const gen::GStage st = gen::make_stage(pl.L, pl.radix[s], Ns);
-    gen::stage<DIR>(A, B, 0, 1, st, tw, pl.tw_stride, lane, 64);
+  for (int s = 0; s < nst; ++s) {
+    const int Rx = uni(pl.radix[s]);
+    const gen::GStage st = gen::make_stage(L, Rx, Ns);
+    gen::stage<DIR>(A, B, 0, 1, st, tw, tws, lane, 64);
     reg::wave_sync();
     float2* tmp = A;
     A = B;
     B = tmp;
-    Ns *= pl.radix[s];
+    Ns *= Rx;
   }
   return A;
+}
+
+__device__ __forceinline__ float uni(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+template <class T>
+__device__ __forceinline__ T* uni(T* p) {
+  const unsigned long long a = reinterpret_cast<unsigned long long>(p);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+  return reinterpret_cast<T*>(((unsigned long long)hi << 32) | lo);
+}
+
+// Geometry made wavefront-uniform (see uni).
+__device__ __forceinline__ Geom geom_uni(const Geom& s) {
+  Geom g;
+  g.R = uni(s.R);
+  g.C = uni(s.C);
+  g.M = uni(s.M);
+  g.H = uni(s.H);
+  g.Lr = uni(s.Lr);
+  g.even = uni(s.even);
+  g.TC = uni(s.TC);
+  g.P = uni(s.P);
+  g.wlast = uni(s.wlast);
+  g.N = (long long)g.R * g.C;
+  g.Hc = (long long)g.R * g.H;
+  g.lgR = uni(s.lgR);
+  return g;
 }
 
 __device__ inline float2* fft_generic(float2* A, float2* B, int nt, int ld,
@@ -83,6 +123,9 @@ __device__ inline float2* fft_generic(float2* A, float2* B, int nt, int ld,
   return dir > 0.f ? fft_block<1>(A, B, nt, ld, pl, tw, tid, nthr)
                    : fft_block<-1>(A, B, nt, ld, pl, tw, tid, nthr);
 }
+
+// Wavefronts of the wavefront-per-row blocks.
+constexpr int kRowWaves = 4;
 
 // Real value n of a row held as complex pairs (even widths) in a padded
 // buffer.
@@ -333,8 +376,6 @@ __device__ inline void row_body(const Geom& g, const FftPlan& rp,
 // The row twiddle table (C values W_C^t) follows the buffers in LDS.  Used
 // when buffers and table fit 64 KiB.
 // --------------------------------------------------------------------------
-constexpr int kRowWaves = 4;
-
 __host__ __device__ inline int row_wave_lds(const Geom& g) {
   return (kRowWaves * 2 * gen::gld(g.Lr) + g.C) * 8;
 }

@@ -60,7 +60,7 @@ class BatchConfig(ctypes.Structure):
     _fields_ = [('col_lds', _i), ('row_lds', _i), ('solve_lanes', _i),
                 ('solve_maps', _i), ('persistent', _i), ('graph', _i),
                 ('tune_ms', ctypes.c_double), ('setup_ms', ctypes.c_double),
-                ('solve_ms', ctypes.c_double)]
+                ('solve_ms', ctypes.c_double), ('branches', _i)]
 
 
 class Info(ctypes.Structure):
@@ -112,6 +112,7 @@ def lib():
         'scc_batched_cbpdn': (_i, [vp, _i, _i, _i, _i, vp, vp, vp, vp]),
         'scc_batched_set_autotune': (_i, [_i]),
         'scc_batched_set_graph': (_i, [_i]),
+        'scc_batched_set_branches': (_i, [_i]),
         'scc_batched_last_config': (_i, [vp]),
         'scc_unrolled_tape': (_i, [vp, _i, _i, vp, _i, _i, _i, _f, _f, _f,
                                    _i, vp, vp]),

@@ -167,3 +167,10 @@ def set_autotune(on):
 def set_graph(on):
     """Replay the iterations from a captured HIP graph (default True)."""
     _abi.lib().scc_batched_set_graph(1 if on else 0)
+
+
+def set_branches(k):
+    """Mode of later calls: None (or k < 0) the tuned mode (default), 0 the
+    work-list kernels, k > 0 per-image kernel chains on min(k, 4) graph
+    branches."""
+    _abi.lib().scc_batched_set_branches(-1 if k is None else int(k))

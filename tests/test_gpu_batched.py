@@ -2,8 +2,10 @@
 FP64 oracle.
 
 Per image the batched result must equal the single-image GPU solve of the
-same image (relative L2 <= 1e-5: the same arithmetic except the order of
-the sums over maps in the X-solve and the panel widths chosen per batch)
+same image (relative L2 <= 1e-5: the same arithmetic except, in the
+work-list mode, the order of the sums over maps in the X-solve and the
+panel widths chosen per batch), in both modes (work lists, per-image
+kernel chains on several graph branches)
 and the FP64 oracle (relative L2 <= max(1e-4, 2 e32), e32 the FP32 oracle's
 own error); with stopping tests and AutoRho every image must stop after the
 same number of iterations as its single-image solve.  Sizes are uneven:
@@ -43,20 +45,27 @@ def _opts(**kw):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize('autotune', [True, False])
-def test_batched_equals_single_and_oracle(gpu, autotune):
+@pytest.mark.parametrize('autotune,branches', [(True, None), (False, None),
+                                               (False, 0), (False, 1),
+                                               (False, 3)])
+def test_batched_equals_single_and_oracle(gpu, autotune, branches):
     import sporco_cuda.cbpdn as cb
     from sporco_cuda import batched
     D, imgs, lm = _problem()
     opt = _opts()
-    print('batched fixed rho', SIZES, 'autotune', autotune, flush=True)
+    print('batched fixed rho', SIZES, 'autotune', autotune, 'branches',
+          branches, flush=True)
     batched.set_autotune(autotune)
+    batched.set_branches(branches)
     try:
         Xb = batched.cbpdn_batched(D, imgs, lm, opt)
     finally:
         batched.set_autotune(True)
+        batched.set_branches(None)
     cfg = batched.last_config()
     assert cfg['graph'] == 1
+    if branches is not None:
+        assert cfg['branches'] == branches
     Dm = np.ascontiguousarray(np.rollaxis(D, 2, 0)).astype(np.float64)
     for S, l, X in zip(imgs, lm, Xb):
         assert X.shape == S.shape + (D.shape[2],)
@@ -72,15 +81,20 @@ def test_batched_equals_single_and_oracle(gpu, autotune):
 
 
 @pytest.mark.gpu
-def test_batched_autorho_and_stopping_per_image(gpu):
+@pytest.mark.parametrize('branches', [0, 2])
+def test_batched_autorho_and_stopping_per_image(gpu, branches):
     import sporco_cuda.cbpdn as cb
     from sporco_cuda import batched
     D, imgs, lm = _problem(seed=1)
     opt = _opts(MaxMainIter=200, rho=None, RelStopTol=2e-3,
                 AutoRho={'Enabled': True, 'Period': 3})
-    print('batched autorho + stop', flush=True)
+    print('batched autorho + stop, branches', branches, flush=True)
     res = []
-    Xb = batched.cbpdn_batched(D, imgs, lm, opt, results=res)
+    batched.set_branches(branches)
+    try:
+        Xb = batched.cbpdn_batched(D, imgs, lm, opt, results=res)
+    finally:
+        batched.set_branches(None)
     its = []
     for S, l, X, r in zip(imgs, lm, Xb, res):
         Xs = cb.cbpdn(D, S, np.float32(l), opt)
@@ -93,7 +107,8 @@ def test_batched_autorho_and_stopping_per_image(gpu):
 
 
 @pytest.mark.gpu
-def test_batched_graph_and_repeat_calls(gpu):
+@pytest.mark.parametrize('branches', [0, 4])
+def test_batched_graph_and_repeat_calls(gpu, branches):
     """Direct launches equal graph replay bit for bit; a second call with
     new lambdas and images reuses the workspace and graph correctly; a per
     map L1Weight is applied per image."""
@@ -103,20 +118,26 @@ def test_batched_graph_and_repeat_calls(gpu):
     M = D.shape[2]
     w = np.linspace(0.5, 1.5, M).astype(np.float32).reshape(1, 1, M)
     opt = _opts(L1Weight=w)
-    print('batched graph / repeat / L1Weight', flush=True)
-    Xg = batched.cbpdn_batched(D, imgs, lm, opt)
-    batched.set_graph(False)
+    print('batched graph / repeat / L1Weight, branches', branches,
+          flush=True)
+    batched.set_branches(branches)
     try:
-        Xd = batched.cbpdn_batched(D, imgs, lm, opt)
-        assert batched.last_config()['graph'] == 0
+        Xg = batched.cbpdn_batched(D, imgs, lm, opt)
+        batched.set_graph(False)
+        try:
+            Xd = batched.cbpdn_batched(D, imgs, lm, opt)
+            assert batched.last_config()['graph'] == 0
+        finally:
+            batched.set_graph(True)
+        for a, b in zip(Xg, Xd):
+            assert np.array_equal(a, b)
+        lm2 = [2 * v for v in lm]
+        imgs2 = [np.ascontiguousarray(np.flipud(S)) for S in imgs]
+        X2 = batched.cbpdn_batched(D, imgs2, lm2, opt)
+        assert batched.last_config()['tune_ms'] == 0.0  # signature cached
+        assert batched.last_config()['branches'] == branches
     finally:
-        batched.set_graph(True)
-    for a, b in zip(Xg, Xd):
-        assert np.array_equal(a, b)
-    lm2 = [2 * v for v in lm]
-    imgs2 = [np.ascontiguousarray(np.flipud(S)) for S in imgs]
-    X2 = batched.cbpdn_batched(D, imgs2, lm2, opt)
-    assert batched.last_config()['tune_ms'] == 0.0  # signature cached
+        batched.set_branches(None)
     for S, l, X in zip(imgs2, lm2, X2):
         Xs = cb.cbpdn(D, S, np.float32(l), opt)
         assert rel(X, Xs) <= 1e-5, rel(X, Xs)
