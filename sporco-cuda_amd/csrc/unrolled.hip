@@ -4,13 +4,13 @@
 // Forward: the solver's own kernels over B images of one size sharing the
 // dictionary (column and row kernels run over B*M maps, the solve takes the
 // image index), fixed rho, no stopping test; (Y, U) is saved every k
-// iterations, and the tape of the last segment is kept with the
+// iterations, and the tape of the last two segments is kept with the
 // checkpoints.  The tape of an iteration is X-hat after the solve (written
 // by the solve kernel) and the soft-threshold support of V = Xr + U as 4
 // bits per coefficient pair (|V| > t and sign V, written by the row
-// kernel).  Backward, per checkpoint segment from last to first: the
-// segment is recomputed from its checkpoint with a tape (not the last one),
-// then every iteration is reversed:
+// kernel).  Backward, per checkpoint segment from last to first: a segment
+// without a kept tape is recomputed from its checkpoint with a tape, then
+// every iteration is reversed:
 //   k_adj_row1   shrink backward (support mask), relaxation backward, and the
 //                forward row transform of Xbar = alpha Vbar
 //   k_col_pow2   forward column transform            (adjoint of the inverse FFT)
@@ -718,28 +718,34 @@ int fwd_iter(UWs* w, const Params& p, float2* xtape, unsigned* tbits,
 }
 
 // Checkpoint buffer: (Y, U) at the start of every segment, then the tape
-// of the last segment (X-hat of each iteration, then its support bits).
+// of the last kTapeSegs segments (X-hat of each of their iterations, then
+// the support bits), which the backward pass reads instead of recomputing
+// them.  Memory stays O(segments x state + kTapeSegs x k x tape).
+constexpr int kTapeSegs = 2;
+
 struct CkptLayout {
-  int nseg, it0_last, cnt_last;
-  size_t tail;  // float offset of the last segment's tape
+  int nseg;
+  int it0_tape, cnt_tape;  // first taped iteration, number of them
+  size_t tail;             // float offset of the tape
 };
 
 CkptLayout ckpt_layout(int n_iter, int k, size_t nr) {
   CkptLayout c;
   c.nseg = (n_iter + k - 1) / k;
-  c.it0_last = (c.nseg - 1) * k;
-  c.cnt_last = n_iter - c.it0_last;
+  c.it0_tape = std::max(0, c.nseg - kTapeSegs) * k;
+  c.cnt_tape = n_iter - c.it0_tape;
   c.tail = (size_t)c.nseg * 2 * nr;
   return c;
 }
 
+// Tape of iteration it0_tape + j.
 float2* tail_xh(const UWs* w, float* ckpt, const CkptLayout& c, int j) {
   return reinterpret_cast<float2*>(ckpt + c.tail) + (size_t)j * w->nc;
 }
 
 unsigned* tail_bits(const UWs* w, float* ckpt, const CkptLayout& c, int j) {
   return reinterpret_cast<unsigned*>(ckpt + c.tail +
-                                     (size_t)c.cnt_last * 2 * w->nc) +
+                                     (size_t)c.cnt_tape * 2 * w->nc) +
          (size_t)j * w->nbits;
 }
 
@@ -758,7 +764,7 @@ size_t scc_unrolled_ckpt_floats(int M, int B, int R, int C, int n_iter,
   const size_t nc = (size_t)B * M * R * (C / 2 + 1);
   const size_t nbits = (size_t)B * M * R * tape_words_per_row(C);
   const CkptLayout c = ckpt_layout(n_iter, ckpt_every, nr);
-  return c.tail + (size_t)c.cnt_last * (2 * nc + nbits);
+  return c.tail + (size_t)c.cnt_tape * (2 * nc + nbits);
 }
 
 int scc_unrolled_forward(const float* D, int M, int K, const float* S, int B,
@@ -796,7 +802,7 @@ int scc_unrolled_forward(const float* D, int M, int K, const float* S, int B,
       io.Uout = io.Yout + nr;
     }
     if (last) io.Yout = Y;
-    const int j = i - c.it0_last;  // tape slot in the last segment
+    const int j = i - c.it0_tape;  // slot in the checkpoint buffer's tape
     const bool tape = ckpt && j >= 0;
     if ((rc = fwd_iter(w, p, tape ? tail_xh(w, ckpt, c, j) : nullptr,
                        tape ? tail_bits(w, ckpt, c, j) : nullptr, nullptr,
@@ -844,18 +850,20 @@ int scc_unrolled_backward(const float* D, int M, int K, const float* S, int B,
   float* lparts = (float*)w->lparts.p;
   int slot = 0;
   int np = 0;
-  // Tape of iteration j of segment sg: the last segment's lives with the
-  // checkpoints, the others' in the workspace (recomputed one at a time).
+  // Tape of iteration j of segment sg: the last kTapeSegs segments' lives
+  // with the checkpoints, the others' in the workspace (recomputed one
+  // segment at a time).
   auto seg_cnt = [&](int sg) {
     return std::min(n_iter, (sg + 1) * ckpt_every) - sg * ckpt_every;
   };
+  auto taped = [&](int sg) { return sg * ckpt_every >= c.it0_tape; };
   auto xh = [&](int sg, int j) {
-    return sg == c.nseg - 1 ? tail_xh(w, ck, c, j)
-                            : (float2*)w->tXh.p + (size_t)j * w->nc;
+    return taped(sg) ? tail_xh(w, ck, c, sg * ckpt_every + j - c.it0_tape)
+                     : (float2*)w->tXh.p + (size_t)j * w->nc;
   };
   auto bits = [&](int sg, int j) {
-    return sg == c.nseg - 1 ? tail_bits(w, ck, c, j)
-                            : (unsigned*)w->tbits.p + (size_t)j * w->nbits;
+    return taped(sg) ? tail_bits(w, ck, c, sg * ckpt_every + j - c.it0_tape)
+                     : (unsigned*)w->tbits.p + (size_t)j * w->nbits;
   };
   // Recompute segment sg from its checkpoint (read in place: rsf is 1, so
   // the row transform of Y - U does not write U) into its own spectra Fr,
@@ -877,7 +885,7 @@ int scc_unrolled_backward(const float* D, int M, int K, const float* S, int B,
     }
     return SCC_OK;
   };
-  if (c.nseg > 1 && (rc = w->Fr.get(w->nc * 8))) return rc;
+  if (!taped(0) && (rc = w->Fr.get(w->nc * 8))) return rc;
   // dL/dY enters through the first shrink backward.
   UHIP(adj_row1(w->gbm, w->cfg, gY, Yb, Ub, bits(c.nseg - 1, seg_cnt(c.nseg - 1) - 1),
                 alpha, b.F, b.twC, lparts + (size_t)slot++ * nparts, &np, st));
@@ -892,7 +900,7 @@ int scc_unrolled_backward(const float* D, int M, int K, const float* S, int B,
       } else if (sg > 0) {
         // The previous segment's tape, then the fused rows across the
         // segment boundary.
-        if ((rc = recompute(sg - 1))) return rc;
+        if (!taped(sg - 1) && (rc = recompute(sg - 1))) return rc;
         UHIP(adj_row21(w->gbm, w->cfg, b.F, Yb, Ub,
                        bits(sg - 1, seg_cnt(sg - 1) - 1), alpha, b.twC,
                        lparts + (size_t)slot++ * nparts, &np, st));

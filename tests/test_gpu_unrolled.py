@@ -117,9 +117,13 @@ def test_unrolled_forward_and_gradients(gpu):
 
 
 @pytest.mark.gpu
-def test_checkpoint_interval_invariance(gpu):
+@pytest.mark.parametrize('ck', [1, 4, 5])
+def test_checkpoint_interval_invariance(gpu, ck):
+    """Recomputed segments (all but the last two) and taped ones give the
+    gradients of a single taped segment."""
     D, S, G = _inputs(3)
-    a = _gpu_grads(D, S, G, 1)
+    print('checkpoint interval', ck, flush=True)
+    a = _gpu_grads(D, S, G, ck)
     b = _gpu_grads(D, S, G, NIT)
     for x, y in zip(a[:3], b[:3]):
         assert rel(x, y) <= 1e-5
