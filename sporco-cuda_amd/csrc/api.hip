@@ -96,7 +96,10 @@ int make_geom(int R, int C, int M, Geom* g, bool* pow2, Pow2Config* p2) {
   if (*pow2) {
     TC = p2->TC;
   } else {
-    TC = gen::col_panel_width(R, g->H, 65536);
+    // 32 KiB of column buffers per block: narrower panels than a 64 KiB
+    // budget allows, but twice the column blocks per CU (measured faster
+    // on uneven sizes, whose column transforms are LDS- and latency-bound).
+    TC = gen::col_panel_width(R, g->H, 32768);
   }
   g->TC = TC;
   g->P = (g->H + TC - 1) / TC;

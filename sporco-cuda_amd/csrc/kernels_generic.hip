@@ -15,6 +15,7 @@
 //   k_reduce          residual norms, stopping test, AutoRho (device side)
 
 #include "generic_common.h"
+#include "scc_pow2.h"
 
 namespace scc {
 
@@ -263,6 +264,7 @@ hipError_t launch_col(const Geom& g, int nmaps, const FftPlan& cp, float2* F,
 
 hipError_t launch_solve(const Geom& g, const Buffers& b, const Params& p,
                         hipStream_t st) {
+  if (solve_regs_fits(g.M)) return solve_regs(g, b, p, st);
   const unsigned nb = (unsigned)((g.Hc + 255) / 256);
   hipLaunchKernelGGL(k_solve_generic, dim3(nb), dim3(256), 0, st, g, b, p);
   return hipGetLastError();

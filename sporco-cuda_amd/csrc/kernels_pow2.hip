@@ -631,6 +631,20 @@ hipError_t pow2_col(const Geom& g, const Pow2Config& cfg, int nmaps,
   return hipErrorNotSupported;
 }
 
+// The solve is independent of the image size and the spectrum layout
+// (frequency f of map m at m Hc + f, GHG at f), so the generic-size solver
+// uses it too for M <= 256: G lanes per frequency, 16 maps per lane.
+bool solve_regs_fits(int M) { return M >= 1 && M <= 256; }
+
+hipError_t solve_regs(const Geom& g, const Buffers& b, const Params& p,
+                      hipStream_t st) {
+  Pow2Config cfg{};
+  int G = 1;
+  while (G * 16 < g.M) G *= 2;
+  cfg.variant = G;
+  return pow2_solve(g, cfg, b, p, st);
+}
+
 hipError_t pow2_solve(const Geom& g, const Pow2Config& cfg, const Buffers& b,
                       const Params& p, hipStream_t st, int nimg, int ctl_iter,
                       int nbsum) {

@@ -43,5 +43,9 @@ hipError_t pow2_solve(const Geom& g, const Pow2Config& cfg, const Buffers& b,
                       int ctl_iter = 0, int nbsum = 0);
 hipError_t pow2_row(const Geom& g, const Pow2Config& cfg, const Buffers& b,
                     const Params& p, int mode, int* nblocks, hipStream_t st);
+// The register-resident X-solve for any image size (one image, M <= 256).
+bool solve_regs_fits(int M);
+hipError_t solve_regs(const Geom& g, const Buffers& b, const Params& p,
+                      hipStream_t st);
 
 }  // namespace scc
