@@ -54,11 +54,23 @@ __host__ __device__ constexpr float kCos16(int m) {
        : m == 14 ? 0.70710678118654752f
        : 0.92387953251128674f;
 }
-// exp(-2 pi i DIR m / N) for N dividing 16.
+// cos(2 pi m / 32), m = 0..31 (the even entries are kCos16(m / 2)).
+__host__ __device__ constexpr float kCos32(int m) {
+  return (m & 1) == 0 ? kCos16(m / 2)
+       : (m == 1 || m == 31)  ? 0.98078528040323044f
+       : (m == 3 || m == 29)  ? 0.83146961230254524f
+       : (m == 5 || m == 27)  ? 0.55557023301960218f
+       : (m == 7 || m == 25)  ? 0.19509032201612826f
+       : (m == 9 || m == 23)  ? -0.19509032201612826f
+       : (m == 11 || m == 21) ? -0.55557023301960218f
+       : (m == 13 || m == 19) ? -0.83146961230254524f
+                              : -0.98078528040323044f;  // 15, 17
+}
+// exp(-2 pi i DIR m / N) for N dividing 32.
 template <int N, int DIR>
 __host__ __device__ constexpr float2 w_const(int m) {
-  return make_float2(kCos16(((m * (16 / N)) % 16)),
-                     (float)(-DIR) * kCos16((((m * (16 / N)) % 16) + 12) % 16));
+  return make_float2(kCos32(((m * (32 / N)) % 32)),
+                     (float)(-DIR) * kCos32((((m * (32 / N)) % 32) + 24) % 32));
 }
 
 // In-place DFT of N = 2, 4, 8 or 16 points, forward (DIR = 1) or
@@ -159,6 +171,14 @@ __host__ __device__ __forceinline__ float2 tw_get(const float2* tw, int e,
   return w;
 }
 
+// W_C^{t + T e} of a real row of C = 2 L points held as T lanes x E slots
+// (C / T = 2 E): the lane's table entry W_C^t times the constant
+// W_{2E}^e.
+template <int E>
+__host__ __device__ __forceinline__ float2 w_slot(float2 wt, int e) {
+  return e == 0 ? wt : c_mul(wt, w_const<2 * E, 1>(e));
+}
+
 // Stage S: twiddle and butterflies on thread t's registers.
 template <int L, int E, int S, int DIR>
 __host__ __device__ __forceinline__ void stage_compute(float2 (&v)[E], int t,
@@ -176,10 +196,15 @@ __host__ __device__ __forceinline__ void stage_compute(float2 (&v)[E], int t,
 #pragma unroll
     for (int r = 0; r < RS; ++r) x[r] = v[b + r * NB];
     if (NS > 1) {
+      // W^{k r STEP} = (W^{k STEP})^r: one table read, the powers by
+      // products of lower powers (at most log2(RS) products deep).
       const int k = (t + T * b) & (NS - 1);
+      float2 pw[RS];
+      pw[1] = tw_get<DIR>(tw, k * STEP, tws);
 #pragma unroll
-      for (int r = 1; r < RS; ++r)
-        x[r] = c_mul(x[r], tw_get<DIR>(tw, k * r * STEP, tws));
+      for (int r = 2; r < RS; ++r) pw[r] = c_mul(pw[r / 2], pw[r - r / 2]);
+#pragma unroll
+      for (int r = 1; r < RS; ++r) x[r] = c_mul(x[r], pw[r]);
     }
     Dft<RS, DIR>::run(x);
 #pragma unroll

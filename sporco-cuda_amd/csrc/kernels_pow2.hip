@@ -297,9 +297,9 @@ k_row_pow2(Geom g, Buffers b, Params p, int mode) {
   // Inverse real-FFT pre-processing:
   // Z'[k] = (X[k] + conj X[L-k]) + i W_C^-k (X[k] - conj X[L-k]).
   float2 z[E];
+  const float2 wt = b.twC[t];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const int k = t + T * e;
     const float2 sh = shfl2(v[E - 1 - e], partner);
     float2 xn = (t == 0) ? (e == 0 ? xL : v[(E - e) % E]) : sh;
     float2 xk = v[e];
@@ -309,7 +309,7 @@ k_row_pow2(Geom g, Buffers b, Params p, int mode) {
     }
     xn.y = -xn.y;
     const float2 s = cadd(xk, xn);
-    const float2 wd = cmul(cconj(b.twC[k]), csub(xk, xn));
+    const float2 wd = cmul(cconj(reg::w_slot<E>(wt, e)), csub(xk, xn));
     z[e] = make_float2(s.x - wd.y, s.y + wd.x);
   }
   reg::fft_wave<LR, E, -1>(z, t, L, b.twC, 2);

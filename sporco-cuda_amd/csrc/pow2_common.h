@@ -69,12 +69,14 @@ __device__ __forceinline__ void store_spectrum(const float2 (&z)[RowCfg<LR>::E],
                                                const float2* twC) {
   using RC = RowCfg<LR>;
   constexpr int E = RC::E, T = RC::T;
+  const float2 wt = twC[t];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int k = t + T * e;
     const float2 sh = shfl2(z[E - 1 - e], partner);
     const float2 zn = (t == 0) ? (e == 0 ? z[0] : z[(E - e) % E]) : sh;
-    Fm[spec_off<TC>(k, r, R)] = r2c_post(z[e], cconj(zn), twC[k]);
+    Fm[spec_off<TC>(k, r, R)] =
+        r2c_post(z[e], cconj(zn), reg::w_slot<E>(wt, e));
   }
   if (t == 0)  // column L: Re Z[0] - Im Z[0]
     Fm[(LR / TC) * TC * R + r] = make_float2(z[0].x - z[0].y, 0.f);
@@ -98,9 +100,9 @@ __device__ __forceinline__ void load_spectrum_c2r(float2 (&z)[RowCfg<LR>::E],
   for (int e = 0; e < E; ++e) v[e] = Fm[spec_off<TC>(t + T * e, r, R)];
   float2 xL = make_float2(0.f, 0.f);
   if (t == 0) xL = Fm[(LR / TC) * TC * R + r];
+  const float2 wt = twC[t];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const int k = t + T * e;
     const float2 sh = shfl2(v[E - 1 - e], partner);
     float2 xn = (t == 0) ? (e == 0 ? xL : v[(E - e) % E]) : sh;
     float2 xk = v[e];
@@ -110,7 +112,7 @@ __device__ __forceinline__ void load_spectrum_c2r(float2 (&z)[RowCfg<LR>::E],
     }
     xn.y = -xn.y;
     const float2 s = cadd(xk, xn);
-    const float2 wd = cmul(cconj(twC[k]), csub(xk, xn));
+    const float2 wd = cmul(cconj(reg::w_slot<E>(wt, e)), csub(xk, xn));
     z[e] = make_float2(s.x - wd.y, s.y + wd.x);
   }
 }

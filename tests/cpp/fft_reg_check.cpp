@@ -1,7 +1,8 @@
 // CPU check of the register-resident Stockham FFT phases (csrc/fft_reg.h):
 // runs stage_compute / stage_write / slots_read for every thread in turn
 // (the device's block barriers become loops) and compares the natural-order
-// result with a direct double-precision DFT.  Prints one line per case and
+// result with a direct double-precision DFT; also checks the real-FFT slot
+// twiddles (lane table entry times a constant root of unity).  Prints one line per case and
 // exits non-zero on failure.
 #include <cmath>
 #include <cstdio>
@@ -101,8 +102,32 @@ int check(int tws) {
   return ok ? 0 : 1;
 }
 
+// w_slot<E>(W_C^t, e) against W_C^{t + T e} in double (C = 2 L, T = L / E).
+template <int L, int E>
+int check_slot_twiddles() {
+  constexpr int T = L / E, C = 2 * L;
+  double err = 0;
+  for (int t = 0; t < T; ++t) {
+    const double a = -2.0 * M_PI * t / C;
+    const float2 wt = make_float2((float)cos(a), (float)sin(a));
+    for (int e = 0; e < E; ++e) {
+      const double b = -2.0 * M_PI * (t + T * e) / C;
+      const float2 w = w_slot<E>(wt, e);
+      err = fmax(err, fabs(w.x - cos(b)) + fabs(w.y - sin(b)));
+    }
+  }
+  const bool ok = err < 5e-7;
+  printf("slot twiddles L=%d E=%d max_err=%.3e %s\n", L, E, err,
+         ok ? "ok" : "FAIL");
+  return ok ? 0 : 1;
+}
+
 int main() {
   int bad = 0;
+  bad += check_slot_twiddles<512, 8>();
+  bad += check_slot_twiddles<128, 8>();
+  bad += check_slot_twiddles<1024, 16>();
+  bad += check_slot_twiddles<2048, 16>();
   bad += check<16, 16, 1>(1);
   bad += check<16, 4, -1>(1);
   bad += check<64, 8, 1>(2);
