@@ -211,9 +211,14 @@ __host__ __device__ __forceinline__ void stage_reg(const float2* A, float2* B,
 #pragma unroll
     for (int r = 0; r < RX; ++r) x[r] = in[gpad(j + r * nb)];
     if (k) {
-      const int e1 = k * s.tstep;
+      // W_L^{e1 r} = (W_L^{e1})^r: one table read, the powers by products
+      // of lower powers (at most log2(RX) + 1 products deep).
+      float2 pw[RX];
+      pw[1] = tw_at<DIR>(tw, k * s.tstep, tws);
 #pragma unroll
-      for (int r = 1; r < RX; ++r) x[r] = c_mul(x[r], tw_at<DIR>(tw, e1 * r, tws));
+      for (int r = 2; r < RX; ++r) pw[r] = c_mul(pw[r / 2], pw[r - r / 2]);
+#pragma unroll
+      for (int r = 1; r < RX; ++r) x[r] = c_mul(x[r], pw[r]);
     }
     bfly<RX, DIR>(x);
     const int ob = (j - k) * RX + k;
