@@ -169,6 +169,7 @@ struct Workspace {
   DevBuf Y, U, F, Df, Sf, ghg, partials, blocksum, ctrl, hist, twR, twC;
   DevBuf sD, sS, sL1, sGW, sMask;  // staging for the host-pointer entry
   DevBuf oG, oZ, oE, oPart, oVal, oHist;  // objective evaluation
+  DevBuf flag;                            // negative-weight test
   bool ghg_ready = false;
   hipStream_t stream = nullptr;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -481,6 +482,20 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
   // Objective evaluation after every iteration (Verbose, or requested).
   const bool obj_on = g_obj_on || o.Verbose == 1;
   const bool aux = o.AuxVarObj == 1;
+  // The state is V (one real array per coefficient) unless the objective
+  // of every iteration needs Y, or a threshold is negative (shrink(0) != 0:
+  // V cannot hold Y = U = 0).
+  bool thr_ok = lambda >= 0.f && (rs.l1_mode != 0 || p.l1_scalar >= 0.f);
+  if (!obj_on && thr_ok && rs.l1_mode != 0) {
+    if ((rc = ensure(ws->flag, 4))) return rc;
+    const long long nw = rs.l1_mode == 1 ? (long long)M : (long long)M * g.N;
+    int neg = 0;
+    SCC_HIP(launch_any_negative(l1w, nw, (int*)ws->flag.p, st));
+    SCC_HIP(hipMemcpyAsync(&neg, ws->flag.p, 4, hipMemcpyDeviceToHost, st));
+    SCC_HIP(hipStreamSynchronize(st));
+    thr_ok = neg == 0;
+  }
+  p.vstate = (!obj_on && thr_ok) ? 1 : 0;
   if (obj_on &&
       ((rc = ensure(ws->oG, (size_t)M * g.Hc * 8)) ||
        (rc = ensure(ws->oZ, aux ? 4 : (size_t)M * g.N * 4)) ||
@@ -615,12 +630,13 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
       SCC_HIP(mark(kStReduce));
     }
     if (rho_step) {
+      const int fm = p.vstate ? kFwdYUtoV : kFwdYU;
       if (ws->pow2)
-        SCC_HIP(pow2_row_fwd(g, ws->p2, M, kFwdYU, nullptr, b.Y, b.U, b.F,
-                             b.twC, b.ctrl, st));
+        SCC_HIP(pow2_row_fwd(g, ws->p2, M, fm, nullptr, b.Y, b.U, b.F, b.twC,
+                             b.ctrl, st, i));
       else
-        SCC_HIP(launch_row_fwd(g, M, ws->rowp, kFwdYU, nullptr, b.Y, b.U, b.F,
-                               b.twC, b.ctrl, st));
+        SCC_HIP(launch_row_fwd(g, M, ws->rowp, fm, nullptr, b.Y, b.U, b.F,
+                               b.twC, b.ctrl, st, i));
       SCC_HIP(mark(kStRowFwd));
     }
     if (obj_on && (rc = objective_step(ws, b, p, aux, i, st))) return rc;
@@ -653,7 +669,9 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
     }
   }
   SCC_HIP(hipEventRecord(ws->ev[2], st));
-  if (Yout != nullptr)
+  if (p.vstate)  // Y = shrink(V), into the output or in place
+    SCC_HIP(launch_state_to_y(g, b, p, Yout != nullptr ? Yout : b.Y, st));
+  else if (Yout != nullptr)
     SCC_HIP(hipMemcpyAsync(Yout, b.Y, nreal * 4, hipMemcpyDeviceToDevice, st));
   SCC_HIP(hipMemcpyAsync(ws->hctrl, b.ctrl, sizeof(Ctrl),
                          hipMemcpyDeviceToHost, st));

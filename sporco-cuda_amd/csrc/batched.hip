@@ -383,10 +383,11 @@ struct BWs {
   bool wave = false;  // wavefront-per-row kernels (every row fits)
   int grid[kNumLists] = {};
   int G = 1;
-  BBuf U, F, Df, Sf, part, ctrl, tw, desc, pre, l1w, bsum;
-  // Chains mode: the iterations run on Yw (same offsets as U), copied to
-  // the caller's outputs afterwards, so that a captured graph holds no
-  // per-call pointer.
+  BBuf U, F, Df, Sf, part, ctrl, tw, desc, pre, l1w, bsum, flag;
+  bool vs = false;  // chains keep the state as V (every threshold >= 0)
+  // Chains mode: the iterations run on the state array Yw (same offsets as
+  // U), turned into the caller's outputs Y afterwards, so that a captured
+  // graph holds no per-call pointer.
   BBuf Yw;
   std::vector<BChain> ch;
   hipStream_t br[kMaxBranches] = {};
@@ -691,11 +692,14 @@ Buffers chain_buffers(const BWs* w, int j) {
   return b;
 }
 
+// The chains keep the state as V (Params::vstate) when every threshold is
+// >= 0.
 Params chain_params(const BWs* w, int j, const Params& p0) {
   Params p = p0;
   p.lambda = w->h[j].lambda;
   p.xi = w->h[j].xi;
   p.sqrt_nc = w->h[j].sqrt_nc;
+  p.vstate = w->vs ? 1 : 0;
   return p;
 }
 
@@ -761,11 +765,11 @@ int chain_iter(BWs* w, int j, const Params& p0, int i, int maxit,
     BHIP(launch_reduce(b, p, nblocks, i, maxit, st));
   if (rho_step) {
     if (c.pow2)
-      BHIP(pow2_row_fwd(g, c.p2, g.M, kFwdYU, nullptr, b.Y, b.U, b.F, b.twC,
-                        b.ctrl, st));
+      BHIP(pow2_row_fwd(g, c.p2, g.M, kFwdYUtoV, nullptr, b.Y, b.U, b.F,
+                        b.twC, b.ctrl, st, i));
     else
-      BHIP(launch_row_fwd(g, g.M, im.rp, kFwdYU, nullptr, b.Y, b.U, b.F,
-                          b.twC, b.ctrl, st));
+      BHIP(launch_row_fwd(g, g.M, im.rp, kFwdYUtoV, nullptr, b.Y, b.U, b.F,
+                          b.twC, b.ctrl, st, i));
   }
   return SCC_OK;
 }
@@ -1031,14 +1035,21 @@ int scc_batched_cbpdn(const float* D, int M, int Kh, int Kw, int nimg,
   p.l1_mode = nl1 == 1 ? 0 : 1;
   p.l1_scalar = 1.f;
   p.gw_scalar = 1.f;
+  int neg = 0;
   if (nl1 == 1) {
     BHIP(hipMemcpyAsync(&p.l1_scalar, o.L1Weight, 4, hipMemcpyDeviceToHost,
                         st));
   } else {
     BHIP(hipMemcpyAsync(w->l1w.p, o.L1Weight, (size_t)M * 4,
                         hipMemcpyDeviceToDevice, st));
+    if ((rc = w->flag.get(4))) return rc;
+    BHIP(launch_any_negative((const float*)w->l1w.p, M, (int*)w->flag.p, st));
+    BHIP(hipMemcpyAsync(&neg, w->flag.p, 4, hipMemcpyDeviceToHost, st));
   }
   BHIP(hipStreamSynchronize(st));
+  // V state needs every threshold (lambda_i / rho) w >= 0.
+  w->vs = neg == 0 && (nl1 != 1 || p.l1_scalar >= 0.f);
+  for (int i = 0; i < nimg; ++i) w->vs = w->vs && images[i].lambda >= 0.f;
 
   // Per-call descriptor fields.
   for (int i = 0; i < nimg; ++i) {
@@ -1116,12 +1127,14 @@ int scc_batched_cbpdn(const float* D, int M, int Kh, int Kw, int nimg,
     std::memcpy(key.data() + sizeof(Params) + sizeof(int), &nl, sizeof(int));
     std::memcpy(key.data() + sizeof(Params) + 2 * sizeof(int), &use.branches,
                 sizeof(int));
-    if (chains)  // the chains' kernels take lambda and xi by value
+    if (chains) {  // the chains' kernels take lambda and xi by value
       for (int i = 0; i < nimg; ++i) {
         const float v[2] = {w->h[i].lambda, w->h[i].xi};
         const unsigned char* b8 = (const unsigned char*)v;
         key.insert(key.end(), b8, b8 + sizeof(v));
       }
+      key.push_back(w->vs ? 1 : 0);
+    }
     if (w->exec == nullptr || key != w->gkey) {
       if (w->exec) {
         (void)hipGraphExecDestroy(w->exec);
@@ -1157,11 +1170,16 @@ int scc_batched_cbpdn(const float* D, int M, int Kh, int Kw, int nimg,
       if ((rc = launch_iter(w, p, i, st))) return rc;
   }
   BHIP(hipEventRecord(w->ev[2], st));
-  if (chains)
-    for (int i = 0; i < nimg; ++i)
-      BHIP(hipMemcpyAsync(images[i].Y, w->h[i].Y,
-                          (size_t)M * w->h[i].g.N * 4,
-                          hipMemcpyDeviceToDevice, st));
+  if (chains)  // Y (= shrink(V) in the V state) into the outputs
+    for (int i = 0; i < nimg; ++i) {
+      if (w->vs)
+        BHIP(launch_state_to_y(w->ch[i].g, chain_buffers(w, i),
+                               chain_params(w, i, p), images[i].Y, st));
+      else
+        BHIP(hipMemcpyAsync(images[i].Y, w->h[i].Y,
+                            (size_t)M * w->h[i].g.N * 4,
+                            hipMemcpyDeviceToDevice, st));
+    }
   BHIP(hipMemcpyAsync(w->hctrl, w->ctrl.p, (size_t)nimg * sizeof(Ctrl),
                       hipMemcpyDeviceToHost, st));
   BHIP(hipStreamSynchronize(st));

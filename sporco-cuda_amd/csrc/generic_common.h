@@ -276,11 +276,13 @@ __device__ inline void row_fwd_body(const Geom& g, long long nrows,
       a = src[gi];
     } else {
       float u = U[gi];
-      if (rsf != 1.f) {
-        u = u / rsf;
+      if (rsf != 1.f) u = u / rsf;
+      const float y = Y[gi];
+      if (mode == kFwdYUtoV)
+        Y[gi] = y + u;  // V with Y = shrink(V), U / rsf = V - Y
+      else if (rsf != 1.f)
         U[gi] = u;
-      }
-      a = Y[gi] - u;
+      a = y - u;
     }
     if (g.even)
       real_at(A + t * ld, n) = a;
@@ -353,7 +355,8 @@ __device__ inline void row_body(const Geom& g, const FftPlan& rp,
     const int r = (int)row - m * g.R;
     const float xv = g.even ? real_at(P + t * ld, n) : P[t * ld + gen::gpad(n)].x;
     const long long gi = row * g.C + n;
-    float a = update_point(g, p, b, xv * invN, lr, m, r, n, gi, acc);
+    float a = update_point(g, p, b, xv * invN, lr, m, r, n, gi, acc,
+                           p.vstate && mode == kRowFused);
     if (mode == kRowFused) {
       if (g.even)
         real_at(P + t * ld, n) = a;
@@ -492,7 +495,8 @@ __device__ inline void row_body_wave(const Geom& g, const FftPlan& rp,
     for (int n = lane; n < C; n += 64) {
       float* slot = g.even ? &real_at(P, n) : &P[gen::gpad(n)].x;
       const float a = update_point(g, p, b, *slot * invN, lr, m, r, n,
-                                   gi0 + n, acc);
+                                   gi0 + n, acc,
+                                   p.vstate && mode == kRowFused);
       if (mode == kRowFused) {
         if (g.even)
           *slot = a;
@@ -538,11 +542,13 @@ __device__ inline void row_fwd_body_wave(const Geom& g, long long nrows,
         a = src[gi];
       } else {
         float u = U[gi];
-        if (rsf != 1.f) {
-          u = u / rsf;
+        if (rsf != 1.f) u = u / rsf;
+        const float y = Y[gi];
+        if (mode == kFwdYUtoV)
+          Y[gi] = y + u;  // V with Y = shrink(V), U / rsf = V - Y
+        else if (rsf != 1.f)
           U[gi] = u;
-        }
-        a = Y[gi] - u;
+        a = y - u;
       }
       if (g.even)
         real_at(A, n) = a;

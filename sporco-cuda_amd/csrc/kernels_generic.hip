@@ -14,6 +14,8 @@
 //                     residual partial sums; Y-U -> row forward FFT
 //   k_reduce          residual norms, stopping test, AutoRho (device side)
 
+#include <algorithm>
+
 #include "generic_common.h"
 #include "scc_pow2.h"
 
@@ -56,10 +58,14 @@ template <bool WAVE>
 __global__ __launch_bounds__(256) void k_row_fwd_generic(
     Geom g, long long nrows, FftPlan rp, int mode, const float* src,
     float* Y, float* U, float2* F, const float2* __restrict__ twC,
-    const Ctrl* ctrl, int nr) {
+    const Ctrl* ctrl, int nr, int iter) {
+  // The V conversion also runs when the stopping test passed at this very
+  // iteration (its row kernel wrote Y and U), so that the state array holds
+  // V whatever iteration the solve stopped at.
   if (mode == kFwdYU && ctrl->done) return;
+  if (mode == kFwdYUtoV && ctrl->done && ctrl->iter != iter) return;
   extern __shared__ float2 lds[];
-  const float rsf = (mode == kFwdYU) ? ctrl->rsf : 1.f;
+  const float rsf = (mode != kFwdReal) ? ctrl->rsf : 1.f;
   if constexpr (WAVE)
     row_fwd_body_wave(g, nrows, rp, mode, src, Y, U, F,
                       row_wave_table(g, lds, twC), rsf, nr, blockIdx.x, lds);
@@ -153,6 +159,24 @@ __global__ __launch_bounds__(256) void k_row_generic(Geom g, FftPlan rp,
 }
 
 // --------------------------------------------------------------------------
+// Y = shrink(V) of every coefficient of a state array holding V
+// (Params::vstate), with the final rho; out may be the state array.
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_state_to_y(Geom g, Buffers b,
+                                                    Params p, float* out) {
+  const float lr = p.lambda / b.ctrl->rho;
+  const long long n = (long long)g.M * g.N;
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n;
+       i += (long long)gridDim.x * 256) {
+    const int m = (int)(i / g.N);
+    const long long rc = i - (long long)m * g.N;
+    const int r = (int)(rc / g.C);
+    const int c = (int)(rc - (long long)r * g.C);
+    out[i] = shrink_coef(g, p, b, b.Y[i], lr, m, r, c, i);
+  }
+}
+
+// --------------------------------------------------------------------------
 // Residual norms, stopping test and AutoRho.  Two deterministic passes:
 // k_reduce_blocks sums contiguous chunks of the partial sums in double (one
 // result per block), k_reduce_final adds the block results in index order
@@ -227,17 +251,17 @@ hipError_t launch_pad(const float* D, float* out, int M, int Kh, int Kw,
 hipError_t launch_row_fwd(const Geom& g, int nmaps, const FftPlan& rp,
                           int mode, const float* src, float* Y, float* U,
                           float2* F, const float2* twC, const Ctrl* ctrl,
-                          hipStream_t st) {
+                          hipStream_t st, int iter) {
   const int nr = row_nr(g);
   const long long nrows = (long long)nmaps * g.R;
   const unsigned nb = (unsigned)((nrows + nr - 1) / nr);
   const size_t shm = row_lds(g, nr);
   if (row_wave_fits(g))
     hipLaunchKernelGGL(k_row_fwd_generic<true>, dim3(nb), dim3(256), shm, st,
-                       g, nrows, rp, mode, src, Y, U, F, twC, ctrl, nr);
+                       g, nrows, rp, mode, src, Y, U, F, twC, ctrl, nr, iter);
   else
     hipLaunchKernelGGL(k_row_fwd_generic<false>, dim3(nb), dim3(256), shm, st,
-                       g, nrows, rp, mode, src, Y, U, F, twC, ctrl, nr);
+                       g, nrows, rp, mode, src, Y, U, F, twC, ctrl, nr, iter);
   return hipGetLastError();
 }
 
@@ -283,6 +307,35 @@ hipError_t launch_row(const Geom& g, const FftPlan& rp, const Buffers& b,
     hipLaunchKernelGGL(k_row_generic<false>, dim3(nb), dim3(256), shm, st, g,
                        rp, b, p, mode, nr);
   nblocks_out[0] = nb;
+  return hipGetLastError();
+}
+
+// flag[0] = 1 when any of the n values is negative (flag cleared first by
+// the single-block launch below, or preset by the caller).
+__global__ __launch_bounds__(256) void k_any_negative(const float* w,
+                                                      long long n, int* flag) {
+  int neg = 0;
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n;
+       i += (long long)gridDim.x * 256)
+    neg |= w[i] < 0.f;
+  if (__any(neg) && (threadIdx.x & 63) == 0) flag[0] = 1;
+}
+
+__global__ void k_clear_flag(int* flag) { flag[0] = 0; }
+
+hipError_t launch_any_negative(const float* w, long long n, int* flag,
+                               hipStream_t st) {
+  hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, st, flag);
+  const unsigned nb = (unsigned)std::min<long long>((n + 255) / 256, 2048);
+  hipLaunchKernelGGL(k_any_negative, dim3(nb), dim3(256), 0, st, w, n, flag);
+  return hipGetLastError();
+}
+
+hipError_t launch_state_to_y(const Geom& g, const Buffers& b,
+                             const Params& p, float* out, hipStream_t st) {
+  const long long n = (long long)g.M * g.N;
+  const unsigned nb = (unsigned)std::min<long long>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_state_to_y, dim3(nb), dim3(256), 0, st, g, b, p, out);
   return hipGetLastError();
 }
 

@@ -25,21 +25,29 @@ namespace {
 constexpr int kColE = 16;  // points per thread in the column transforms
 
 // One element of the fused update on a value x (= X, already / N).
+// Y from V (general update rules).
+__device__ __forceinline__ float shrink1(const Params& p, float v, float thr,
+                                         int amsw, bool ams, bool zero_bndry) {
+  float y;
+  if (ams) {
+    y = (float)(1 - amsw) * v;
+  } else {
+    y = copysignf(fmaxf(fabsf(v) - thr, 0.f), v);
+    if (p.nonneg) y = fmaxf(y, 0.f);
+    if (zero_bndry) y = 0.f;
+  }
+  return y;
+}
+
 __device__ __forceinline__ void upd1(const Params& p, float x, float yo,
                                      float uo, float thr, int amsw,
                                      bool ams, bool zero_bndry, float& yn,
-                                     float& un, UpdateSums& acc) {
+                                     float& un, float& vv, UpdateSums& acc) {
   const float xr = (p.alpha == 1.f) ? x : p.alpha * x + (1.f - p.alpha) * yo;
   const float v = xr + uo;
-  if (ams) {
-    yn = (float)(1 - amsw) * v;
-  } else {
-    const float a = fabsf(v) - thr;
-    yn = copysignf(fmaxf(a, 0.f), v);
-    if (p.nonneg) yn = fmaxf(yn, 0.f);
-    if (zero_bndry) yn = 0.f;
-  }
+  yn = shrink1(p, v, thr, amsw, ams, zero_bndry);
   un = v - yn;
+  vv = v;
   acc.x += x * x;
   acc.y += yn * yn;
   acc.u += un * un;
@@ -243,7 +251,10 @@ __global__ __launch_bounds__(256) void k_solve_pow2(Geom g, Buffers b,
 // --------------------------------------------------------------------------
 // TAPE (plain update only) records the unrolled solver's tape: 1 the
 // support bits of V (|V| > t, sign V) to b.tbits, 2 V itself to b.vout.
-template <int LR, int TC, bool GENERAL, int TAPE>
+// VS: the state array holds V (Params::vstate); the old Y and U are decoded
+// from it, and a fused launch stores the new V (an update-only launch
+// stores Y and U).
+template <int LR, int TC, bool GENERAL, int TAPE, bool VS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
     GENERAL ? 1 : RowCfg<LR>::WAVES))) void
 k_row_pow2(Geom g, Buffers b, Params p, int mode) {
@@ -291,7 +302,7 @@ k_row_pow2(Geom g, Buffers b, Params p, int mode) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       y2[e] = Y2i[lo + T * e];
-      u2[e] = U2i[lo + T * e];
+      if (!VS) u2[e] = U2i[lo + T * e];
     }
   }
   // Inverse real-FFT pre-processing:
@@ -327,9 +338,10 @@ k_row_pow2(Geom g, Buffers b, Params p, int mode) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       y2[e] = Y2i[lo + T * e];
-      u2[e] = U2i[lo + T * e];
+      if (!VS) u2[e] = U2i[lo + T * e];
     }
   }
+  const bool store_v = VS && mode == kRowFused;
   if (GENERAL) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -349,13 +361,25 @@ k_row_pow2(Geom g, Buffers b, Params p, int mode) {
       }
       const bool zb0 = p.nobndry && (bndry_row || n0 > C - p.Kw);
       const bool zb1 = p.nobndry && (bndry_row || n0 + 1 > C - p.Kw);
-      float yn0, un0, yn1, un1;
-      upd1(p, z[e].x * invN, y2[e].x, u2[e].x, th0, w0, ams, zb0, yn0, un0,
-           acc);
-      upd1(p, z[e].y * invN, y2[e].y, u2[e].y, th1, w1, ams, zb1, yn1, un1,
-           acc);
-      Y2[lo + T * e] = make_float2(yn0, yn1);
-      U2[lo + T * e] = make_float2(un0, un1);
+      float yo0 = y2[e].x, yo1 = y2[e].y, uo0, uo1;
+      if (VS) {  // y2 holds V
+        yo0 = shrink1(p, y2[e].x, th0, w0, ams, zb0);
+        yo1 = shrink1(p, y2[e].y, th1, w1, ams, zb1);
+        uo0 = y2[e].x - yo0;
+        uo1 = y2[e].y - yo1;
+      } else {
+        uo0 = u2[e].x;
+        uo1 = u2[e].y;
+      }
+      float yn0, un0, yn1, un1, vv0, vv1;
+      upd1(p, z[e].x * invN, yo0, uo0, th0, w0, ams, zb0, yn0, un0, vv0, acc);
+      upd1(p, z[e].y * invN, yo1, uo1, th1, w1, ams, zb1, yn1, un1, vv1, acc);
+      if (store_v) {
+        Y2[lo + T * e] = make_float2(vv0, vv1);
+      } else {
+        Y2[lo + T * e] = make_float2(yn0, yn1);
+        U2[lo + T * e] = make_float2(un0, un1);
+      }
       z[e] = make_float2(yn0 - un0, yn1 - un1);
     }
   } else {
@@ -370,8 +394,18 @@ k_row_pow2(Geom g, Buffers b, Params p, int mode) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const float x0 = z[e].x * invN, x1 = z[e].y * invN;
-      const float v0 = fmaf(a, x0, fmaf(a1, y2[e].x, u2[e].x));
-      const float v1 = fmaf(a, x1, fmaf(a1, y2[e].y, u2[e].y));
+      float yo0 = y2[e].x, yo1 = y2[e].y, uo0, uo1;
+      if (VS) {  // y2 holds V
+        yo0 = copysignf(fmaxf(fabsf(y2[e].x) - th, 0.f), y2[e].x);
+        yo1 = copysignf(fmaxf(fabsf(y2[e].y) - th, 0.f), y2[e].y);
+        uo0 = y2[e].x - yo0;
+        uo1 = y2[e].y - yo1;
+      } else {
+        uo0 = u2[e].x;
+        uo1 = u2[e].y;
+      }
+      const float v0 = fmaf(a, x0, fmaf(a1, yo0, uo0));
+      const float v1 = fmaf(a, x1, fmaf(a1, yo1, uo1));
       const float yn0 = copysignf(fmaxf(fabsf(v0) - th, 0.f), v0);
       const float yn1 = copysignf(fmaxf(fabsf(v1) - th, 0.f), v1);
       const float un0 = v0 - yn0, un1 = v1 - yn1;
@@ -389,10 +423,14 @@ k_row_pow2(Geom g, Buffers b, Params p, int mode) {
       acc.u = fmaf(un0, un0, fmaf(un1, un1, acc.u));
       const float d0 = x0 - yn0, d1 = x1 - yn1;
       acc.r = fmaf(d0, d0, fmaf(d1, d1, acc.r));
-      const float s0 = y2[e].x - yn0, s1 = y2[e].y - yn1;
+      const float s0 = yo0 - yn0, s1 = yo1 - yn1;
       acc.s = fmaf(s0, s0, fmaf(s1, s1, acc.s));
-      Y2[lo + T * e] = make_float2(yn0, yn1);
-      U2[lo + T * e] = make_float2(un0, un1);
+      if (store_v) {
+        Y2[lo + T * e] = make_float2(v0, v1);
+      } else {
+        Y2[lo + T * e] = make_float2(yn0, yn1);
+        U2[lo + T * e] = make_float2(un0, un1);
+      }
       z[e] = make_float2(yn0 - un0, yn1 - un1);
     }
     if (TAPE == 1) {
@@ -420,14 +458,18 @@ k_row_pow2(Geom g, Buffers b, Params p, int mode) {
 template <int LR, int TC>
 __global__ __launch_bounds__(256) void k_row_fwd_pow2(
     Geom g, int mode, const float* __restrict__ src, float* Y, float* U,
-    float2* __restrict__ F, const float2* __restrict__ twC,
-    const Ctrl* ctrl) {
+    float2* __restrict__ F, const float2* __restrict__ twC, const Ctrl* ctrl,
+    int iter) {
   using RC = RowCfg<LR>;
   constexpr int E = RC::E, T = RC::T, RPW = RC::RPW;
   constexpr int C = 2 * LR;
   constexpr int LP = reg::lds_len(LR);
   __shared__ float2 lds[4 * RPW * LP];
+  // The V conversion also runs when the stopping test passed at this very
+  // iteration (its row kernel wrote Y and U), so that the state array holds
+  // V whatever iteration the solve stopped at.
   if (mode == kFwdYU && ctrl->done) return;
+  if (mode == kFwdYUtoV && ctrl->done && ctrl->iter != iter) return;
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int rw = lane / T;
@@ -447,16 +489,17 @@ __global__ __launch_bounds__(256) void k_row_fwd_pow2(
     for (int e = 0; e < E; ++e) z[e] = S2[lo + T * e];
   } else {
     const float rsf = ctrl->rsf;
-    const float2* Y2 = reinterpret_cast<const float2*>(Y + rowoff);
+    float2* Y2 = reinterpret_cast<float2*>(Y + rowoff);
     float2* U2 = reinterpret_cast<float2*>(U + rowoff);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const float2 y = Y2[lo + T * e];
       float2 u = U2[lo + T * e];
-      if (rsf != 1.f) {
-        u = make_float2(u.x / rsf, u.y / rsf);
+      if (rsf != 1.f) u = make_float2(u.x / rsf, u.y / rsf);
+      if (mode == kFwdYUtoV)  // V with Y = shrink(V), U / rsf = V - Y
+        Y2[lo + T * e] = make_float2(y.x + u.x, y.y + u.y);
+      else if (rsf != 1.f)
         U2[lo + T * e] = u;
-      }
       z[e] = make_float2(y.x - u.x, y.y - u.y);
     }
   }
@@ -514,18 +557,24 @@ hipError_t row_launch(const Geom& g, const Buffers& b, const Params& p,
                       int mode, int* nblocks, hipStream_t st) {
   const unsigned nb = (unsigned)((long long)g.M * g.R / RowCfg<LR>::RPB);
   const bool general = p.m0 || p.l1_mode == 2 || p.nonneg || p.nobndry;
-  if (general)
-    hipLaunchKernelGGL((k_row_pow2<LR, TC, true, 0>), dim3(nb), dim3(256), 0,
-                       st, g, b, p, mode);
+  if (general && p.vstate)
+    hipLaunchKernelGGL((k_row_pow2<LR, TC, true, 0, true>), dim3(nb),
+                       dim3(256), 0, st, g, b, p, mode);
+  else if (general)
+    hipLaunchKernelGGL((k_row_pow2<LR, TC, true, 0, false>), dim3(nb),
+                       dim3(256), 0, st, g, b, p, mode);
   else if (b.tbits != nullptr)
-    hipLaunchKernelGGL((k_row_pow2<LR, TC, false, 1>), dim3(nb), dim3(256), 0,
-                       st, g, b, p, mode);
+    hipLaunchKernelGGL((k_row_pow2<LR, TC, false, 1, false>), dim3(nb),
+                       dim3(256), 0, st, g, b, p, mode);
   else if (b.vout != nullptr)
-    hipLaunchKernelGGL((k_row_pow2<LR, TC, false, 2>), dim3(nb), dim3(256), 0,
-                       st, g, b, p, mode);
+    hipLaunchKernelGGL((k_row_pow2<LR, TC, false, 2, false>), dim3(nb),
+                       dim3(256), 0, st, g, b, p, mode);
+  else if (p.vstate)
+    hipLaunchKernelGGL((k_row_pow2<LR, TC, false, 0, true>), dim3(nb),
+                       dim3(256), 0, st, g, b, p, mode);
   else
-    hipLaunchKernelGGL((k_row_pow2<LR, TC, false, 0>), dim3(nb), dim3(256), 0,
-                       st, g, b, p, mode);
+    hipLaunchKernelGGL((k_row_pow2<LR, TC, false, 0, false>), dim3(nb),
+                       dim3(256), 0, st, g, b, p, mode);
   *nblocks = (int)nb * 4;  // partial sums per wavefront
   return hipGetLastError();
 }
@@ -533,10 +582,10 @@ hipError_t row_launch(const Geom& g, const Buffers& b, const Params& p,
 template <int LR, int TC>
 hipError_t row_fwd_launch(const Geom& g, int nmaps, int mode, const float* src,
                           float* Y, float* U, float2* F, const float2* twC,
-                          const Ctrl* ctrl, hipStream_t st) {
+                          const Ctrl* ctrl, hipStream_t st, int iter) {
   const unsigned nb = (unsigned)((long long)nmaps * g.R / RowCfg<LR>::RPB);
   hipLaunchKernelGGL((k_row_fwd_pow2<LR, TC>), dim3(nb), dim3(256), 0, st, g,
-                     mode, src, Y, U, F, twC, ctrl);
+                     mode, src, Y, U, F, twC, ctrl, iter);
   return hipGetLastError();
 }
 
@@ -581,12 +630,12 @@ hipError_t pow2_row(const Geom& g, const Pow2Config& cfg, const Buffers& b,
 hipError_t pow2_row_fwd(const Geom& g, const Pow2Config& cfg, int nmaps,
                         int mode, const float* src, float* Y, float* U,
                         float2* F, const float2* twC, const Ctrl* ctrl,
-                        hipStream_t st) {
+                        hipStream_t st, int iter) {
   const int LR = g.C / 2;
 #define X(lr, tc)                                                           \
   if (LR == lr && cfg.TC == tc)                                             \
     return row_fwd_launch<lr, tc>(g, nmaps, mode, src, Y, U, F, twC, ctrl,  \
-                                  st);
+                                  st, iter);
   SCC_ROW_CASES(X)
 #undef X
   return hipErrorNotSupported;

@@ -62,14 +62,13 @@ struct UpdateSums {
 // plus the five residual sums of cbpdn_kernels.cu:695-783 (X unrelaxed,
 // Yprv = Yold, so no copy of the previous Y is kept).  Returns Y - U, the
 // input of the next iteration's forward transform.
-__device__ __forceinline__ float update_point(const Geom& g, const Params& p,
-                                              const Buffers& b, float x,
-                                              float lr, int m, int r, int n,
-                                              long long gi, UpdateSums& acc) {
-  const float yo = b.Y[gi];
-  const float uo = b.U[gi];
-  const float xr = (p.alpha == 1.f) ? x : p.alpha * x + (1.f - p.alpha) * yo;
-  const float v = xr + uo;
+// Y of coefficient (m, r, n) from V = Xr + U: the soft threshold
+// (lambda/rho) w with the NonNegCoef / NoBndryCross rules, or (1 - W) V on
+// the AMS impulse map.
+__device__ __forceinline__ float shrink_coef(const Geom& g, const Params& p,
+                                             const Buffers& b, float v,
+                                             float lr, int m, int r, int n,
+                                             long long gi) {
   float yn;
   if (m < p.m0) {
     yn = (float)(1 - b.mask[(long long)r * g.C + n]) * v;
@@ -81,9 +80,35 @@ __device__ __forceinline__ float update_point(const Geom& g, const Params& p,
     if (p.nonneg) yn = fmaxf(yn, 0.f);
     if (p.nobndry && (r > g.R - p.Kh || n > g.C - p.Kw)) yn = 0.f;
   }
+  return yn;
+}
+
+// store_v (state V, fused row launch): the new V is stored instead of Y and
+// U.  With p.vstate the old Y and U are decoded from the stored V.
+__device__ __forceinline__ float update_point(const Geom& g, const Params& p,
+                                              const Buffers& b, float x,
+                                              float lr, int m, int r, int n,
+                                              long long gi, UpdateSums& acc,
+                                              bool store_v = false) {
+  float yo, uo;
+  if (p.vstate) {
+    const float vo = b.Y[gi];
+    yo = shrink_coef(g, p, b, vo, lr, m, r, n, gi);
+    uo = vo - yo;
+  } else {
+    yo = b.Y[gi];
+    uo = b.U[gi];
+  }
+  const float xr = (p.alpha == 1.f) ? x : p.alpha * x + (1.f - p.alpha) * yo;
+  const float v = xr + uo;
+  const float yn = shrink_coef(g, p, b, v, lr, m, r, n, gi);
   const float un = v - yn;
-  b.Y[gi] = yn;
-  b.U[gi] = un;
+  if (store_v) {
+    b.Y[gi] = v;
+  } else {
+    b.Y[gi] = yn;
+    b.U[gi] = un;
+  }
   acc.x += x * x;
   acc.y += yn * yn;
   acc.u += un * un;

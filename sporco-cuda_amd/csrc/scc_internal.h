@@ -76,6 +76,15 @@ struct Params {
   int grad;             // gradient regularisation on
   int gw_mode;          // 0 scalar, 1 per map
   int track;            // record per-iteration history
+  // 1: the state array (Buffers::Y) holds V = Xr + U of the last update
+  // instead of Y and U, which are Y = shrink(V), U = V - Y; the row kernel
+  // then reads and writes one real array per coefficient instead of two.
+  // An update-only row launch (AutoRho step) writes Y and U, and the
+  // forward row transform of kFwdYUtoV turns them back into V.  Valid
+  // when every threshold (lambda/rho) w is >= 0: then shrink(0) = 0, so V =
+  // 0 is the initial Y = U = 0, and V' = Y + U/rsf decodes to (Y, U/rsf)
+  // after a rho update.
+  int vstate;
 };
 
 // Row-kernel modes
@@ -86,6 +95,7 @@ enum RowMode {
 enum RowFwdMode {
   kFwdReal = 0,     // forward row FFT of a real (rows, C) array
   kFwdYU = 1,       // forward row FFT of Y - U/rsf (U rescaled in place)
+  kFwdYUtoV = 2,    // the same transform; Y replaced by V = Y + U/rsf
 };
 
 struct Buffers {
@@ -114,10 +124,12 @@ struct Buffers {
 // Kernel launchers (kernels.hip).
 hipError_t launch_pad(const float* D, float* out, int M, int Kh, int Kw,
                       int R, int C, hipStream_t st);
+// iter: the iteration of a kFwdYUtoV launch (its conversion is skipped
+// when the solve stopped at an earlier iteration).
 hipError_t launch_row_fwd(const Geom& g, int nmaps, const FftPlan& rp,
                           int mode, const float* src, float* Y, float* U,
                           float2* F, const float2* twC, const Ctrl* ctrl,
-                          hipStream_t st);
+                          hipStream_t st, int iter = 0);
 hipError_t launch_col(const Geom& g, int nmaps, const FftPlan& cp,
                       float2* F, const float2* twR, int inverse,
                       const Ctrl* ctrl, hipStream_t st);
@@ -129,6 +141,12 @@ hipError_t launch_row(const Geom& g, const FftPlan& rp, const Buffers& b,
 hipError_t launch_reduce(const Buffers& b, const Params& p, int nblocks,
                          int iter, int max_iter, hipStream_t st);
 int row_blocks(const Geom& g);
+// flag[0] = 1 if any of the n device values is negative, else 0.
+hipError_t launch_any_negative(const float* w, long long n, int* flag,
+                               hipStream_t st);
+// Y = shrink(V) of a state array holding V (Params::vstate) into out.
+hipError_t launch_state_to_y(const Geom& g, const Buffers& b,
+                             const Params& p, float* out, hipStream_t st);
 
 // Objective kernels (objective.hip): per-frequency fidelity / gradient
 // sums over the spectra G of the objective variable (E != nullptr: store the

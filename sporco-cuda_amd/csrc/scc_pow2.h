@@ -18,7 +18,7 @@ int pow2_row_blocks(const Geom& g, const Pow2Config& cfg);
 hipError_t pow2_row_fwd(const Geom& g, const Pow2Config& cfg, int nmaps,
                         int mode, const float* src, float* Y, float* U,
                         float2* F, const float2* twC, const Ctrl* ctrl,
-                        hipStream_t st);
+                        hipStream_t st, int iter = 0);
 // Inverse real row transforms of nmaps maps of half spectra into real
 // (nmaps, R, C) rows, times scale.
 hipError_t pow2_row_inv(const Geom& g, const Pow2Config& cfg, int nmaps,
