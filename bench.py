@@ -17,7 +17,10 @@ The JSON line carries:
   cpu_baseline  the FP32 numpy/scipy restatement of the same loop (the
                 oracle) timed on the host for a bounded number of iterations;
   iteration_model  the whole iteration against the 4-stage compulsory
-                traffic model 32 M N + 40 M Hc + 12 Hc bytes.
+                traffic model 32 M N + 40 M Hc + 12 Hc bytes (the solver
+                moves 8 M N + 72 M Hc + 8 Hc: its state is V = Xr + U, one
+                real array, instead of Y and U).
+  kernels       per-kernel average duration and algorithmic GB/s.
 """
 
 import argparse
@@ -90,7 +93,7 @@ def stage_bytes(M, R, C):
         'col_fwd': 16 * M * Hc,                 # read + write the spectra
         'solve': 24 * M * Hc + 8 * Hc,          # A, D spectra in, X out, S
         'col_inv': 16 * M * Hc,
-        'row': 16 * M * N + 16 * M * Hc,        # X in, Y U in/out, A out
+        'row': 8 * M * N + 16 * M * Hc,         # X in, V in/out, A out
     }
 
 
