@@ -162,3 +162,26 @@ def test_batched_uneven_multiples_of_eight(gpu):
     for S, l, X in zip(imgs, lm, Xb):
         Xs = cb.cbpdn(D, S, np.float32(l), opt)
         assert rel(X, Xs) <= 1e-5, rel(X, Xs)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('branches', [0, 2])
+def test_batched_negative_weight_keeps_two_array_state(gpu, branches):
+    """A negative per-map L1 weight (shrink(0) != 0, so the state cannot be
+    V = Xr + U) in both modes: the result still equals the single-image
+    solve of every image."""
+    import sporco_cuda.cbpdn as cb
+    from sporco_cuda import batched
+    D, imgs, lm = _problem(seed=4)
+    M = D.shape[2]
+    w = np.linspace(1.5, -0.2, M).astype(np.float32).reshape(1, 1, M)
+    opt = _opts(L1Weight=w, MaxMainIter=20)
+    print('batched negative weight, branches', branches, flush=True)
+    batched.set_branches(branches)
+    try:
+        Xb = batched.cbpdn_batched(D, imgs, lm, opt)
+    finally:
+        batched.set_branches(None)
+    for S, l, X in zip(imgs, lm, Xb):
+        Xs = cb.cbpdn(D, S, np.float32(l), opt)
+        assert rel(X, Xs) <= 1e-5, rel(X, Xs)
