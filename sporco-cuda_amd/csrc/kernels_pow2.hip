@@ -94,7 +94,7 @@ __device__ __forceinline__ void partials_slice(const float* __restrict__ parts,
 // Column transforms: one panel (TC columns x R rows, row-major, contiguous
 // in HBM) per block, E = 16 points per thread.  With nparts > 0 the first
 // nred blocks (in dispatch order) also run the first pass of the previous
-// row kernel's residual reduction (partials_slice) after their own stores.
+// row kernel's residual reduction (partials_slice) before their transforms.
 // --------------------------------------------------------------------------
 template <int R, int TC, int DIR>
 __global__ __launch_bounds__(TC*(R / kColE)) void k_col_pow2(
@@ -111,6 +111,11 @@ __global__ __launch_bounds__(TC*(R / kColE)) void k_col_pow2(
   __shared__ float2 lds[TC * LP];
   __shared__ double red[5][16];
   if (ctrl != nullptr && ctrl->done) return;
+  // The previous row kernel's partials are complete at launch, so the
+  // reducing blocks add their slices first and overlap the others' FFTs.
+  const int lin = blockIdx.y * gridDim.x + blockIdx.x;
+  if (nparts > 0 && lin < nred)
+    partials_slice(parts, nparts, bsum, lin, nred, red);
   const int p = blockIdx.x;
   const long long m = blockIdx.y;
   const int w = (p == g.P - 1) ? g.wlast : TC;
@@ -132,9 +137,6 @@ __global__ __launch_bounds__(TC*(R / kColE)) void k_col_pow2(
 #pragma unroll
     for (int e = 0; e < E; ++e) base[(t + T * e) * w + c] = v[e];
   }
-  const int lin = blockIdx.y * gridDim.x + blockIdx.x;
-  if (nparts > 0 && lin < nred)
-    partials_slice(parts, nparts, bsum, lin, nred, red);
 }
 
 // --------------------------------------------------------------------------
