@@ -584,18 +584,20 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
   const bool can_stop = o.RelStopTol > 0.f || o.AbsStopTol > 0.f;
   const int poll = o.Verbose == 1 ? 1 : 16;
   int nblocks = 0;
-  // Power-of-two path: the residual reduction of an iteration that is not
-  // an AutoRho step, a host poll or the last one is deferred into the next
-  // iteration (first pass in the column kernel's blocks, second pass and
-  // control update in block 0 of the solve, before the inverse transforms
-  // and the row kernel test the stopping flag), saving two launches.
+  // With the register solve (power-of-two sizes, or M <= 256) the residual
+  // reduction of an iteration that is not an AutoRho step, a host poll or
+  // the last one is deferred into the next iteration (first pass in the
+  // column kernel's blocks, second pass and control update in block 0 of
+  // the solve, before the inverse transforms and the row kernel test the
+  // stopping flag), saving two launches.
   bool pending = false;
   for (int i = 1; i <= maxit; ++i) {
     const bool rho_step = p.autorho && i > 1 && i % p.period == 0;
     const int rmode = rho_step ? kRowUpdate : kRowFused;
     const bool poll_pt =
         (can_stop || o.Verbose == 1) && (i % poll == 0 || i == maxit);
-    const bool defer = ws->pow2 && !rho_step && !poll_pt && i < maxit;
+    const bool defer = (ws->pow2 || solve_regs_fits(M)) && !rho_step &&
+                       !poll_pt && i < maxit;
     SCC_HIP(mark(-1));
     if (ws->pow2) {
       SCC_HIP(pow2_col(g, ws->p2, M, b.F, b.twR, 0, b.ctrl, st,
@@ -612,9 +614,12 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
       SCC_HIP(mark(kStColInv));
       SCC_HIP(pow2_row(g, ws->p2, b, p, rmode, &nblocks, st));
     } else {
-      SCC_HIP(launch_col(g, M, ws->colp, b.F, b.twR, 0, b.ctrl, st));
+      SCC_HIP(launch_col(g, M, ws->colp, b.F, b.twR, 0, b.ctrl, st,
+                         pending ? b.partials : nullptr, pending ? nblocks : 0,
+                         b.blocksum));
       SCC_HIP(mark(kStColFwd));
-      SCC_HIP(launch_solve(g, b, p, st));
+      SCC_HIP(launch_solve(g, b, p, st, pending ? i - 1 : 0,
+                           pending ? pow2_defer_blocks(g, M) : 0));
       SCC_HIP(mark(kStSolve));
       if (obj_on && !aux)  // spectrum of X for the objective
         SCC_HIP(hipMemcpyAsync(ws->oG.p, b.F, (size_t)M * g.Hc * 8,

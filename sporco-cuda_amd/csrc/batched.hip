@@ -733,7 +733,7 @@ int chain_setup(BWs* w, int j, hipStream_t st) {
 // last one.
 bool chain_defers(const BChain& c, const Params& p, int i, int maxit) {
   const bool rho_step = p.autorho && i > 1 && i % p.period == 0;
-  return c.pow2 && !rho_step && i < maxit;
+  return (c.pow2 || solve_regs_fits(c.g.M)) && !rho_step && i < maxit;
 }
 
 int chain_iter(BWs* w, int j, const Params& p0, int i, int maxit,
@@ -756,8 +756,11 @@ int chain_iter(BWs* w, int j, const Params& p0, int i, int maxit,
     BHIP(pow2_col(g, c.p2, g.M, b.F, b.twR, 1, b.ctrl, st));
     BHIP(pow2_row(g, c.p2, b, p, rmode, &nblocks, st));
   } else {
-    BHIP(launch_col(g, g.M, im.cp, b.F, b.twR, 0, b.ctrl, st));
-    BHIP(launch_solve(g, b, p, st));
+    BHIP(launch_col(g, g.M, im.cp, b.F, b.twR, 0, b.ctrl, st,
+                    pending ? b.partials : nullptr, pending ? c.nblk : 0,
+                    b.blocksum));
+    BHIP(launch_solve(g, b, p, st, pending ? i - 1 : 0,
+                      pending ? pow2_defer_blocks(g, g.M) : 0));
     BHIP(launch_col(g, g.M, im.cp, b.F, b.twR, 1, b.ctrl, st));
     BHIP(launch_row(g, im.rp, b, p, rmode, &nblocks, st));
   }
@@ -888,7 +891,7 @@ int autotune(BWs* w, const Params& p0, hipStream_t st) {
     }
   }
   // Work lists against per-image chains on 1, 2 or 4 branches, over two
-  // iterations (with the power-of-two images' deferred reduction).
+  // iterations (with the chains' deferred reduction).
   if ((rc = configure(w, best, st)) || (rc = reset())) return rc;
   if ((rc = time_it(w, st, 2,
                     [&] {

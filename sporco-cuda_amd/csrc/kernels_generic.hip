@@ -85,11 +85,18 @@ __global__ __launch_bounds__(256) void k_row_inv_generic(
 // --------------------------------------------------------------------------
 // Column transforms, one panel of one map per block, in place.
 // --------------------------------------------------------------------------
+// With nparts > 0 the first nred blocks first add slices of the previous
+// row kernel's residual partials (first pass of a deferred reduction).
 __global__ __launch_bounds__(256) void k_col_generic(
     Geom g, FftPlan cp, float2* F, const float2* __restrict__ twR,
-    float dir, const Ctrl* ctrl) {
+    float dir, const Ctrl* ctrl, const float* __restrict__ parts, int nparts,
+    double* bsum, int nred) {
   if (ctrl != nullptr && ctrl->done) return;
   extern __shared__ float2 lds[];
+  __shared__ double red[5][16];
+  const int lin = blockIdx.y * gridDim.x + blockIdx.x;
+  if (nparts > 0 && lin < nred)
+    partials_slice(parts, nparts, bsum, lin, nred, red);
   col_body(g, cp, F, twR, dir, blockIdx.x, blockIdx.y, lds);
 }
 
@@ -282,16 +289,20 @@ hipError_t launch_row_inv(const Geom& g, int nmaps, const FftPlan& rp,
 
 hipError_t launch_col(const Geom& g, int nmaps, const FftPlan& cp, float2* F,
                       const float2* twR, int inverse, const Ctrl* ctrl,
-                      hipStream_t st) {
+                      hipStream_t st, const float* parts, int nparts,
+                      double* bsum) {
   const size_t shm = (size_t)2 * g.TC * gen::gld(g.R) * sizeof(float2);
+  const int nred = nparts > 0 ? pow2_defer_blocks(g, nmaps) : 0;
   hipLaunchKernelGGL(k_col_generic, dim3(g.P, nmaps), dim3(256), shm, st, g,
-                     cp, F, twR, inverse ? -1.f : 1.f, ctrl);
+                     cp, F, twR, inverse ? -1.f : 1.f, ctrl, parts, nparts,
+                     bsum, nred);
   return hipGetLastError();
 }
 
 hipError_t launch_solve(const Geom& g, const Buffers& b, const Params& p,
-                        hipStream_t st) {
-  if (solve_regs_fits(g.M)) return solve_regs(g, b, p, st);
+                        hipStream_t st, int ctl_iter, int nbsum) {
+  if (solve_regs_fits(g.M)) return solve_regs(g, b, p, st, ctl_iter, nbsum);
+  if (ctl_iter > 0) return hipErrorInvalidValue;  // no deferral here
   const unsigned nb = (unsigned)((g.Hc + 255) / 256);
   hipLaunchKernelGGL(k_solve_generic, dim3(nb), dim3(256), 0, st, g, b, p);
   return hipGetLastError();

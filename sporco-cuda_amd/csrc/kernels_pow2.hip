@@ -57,36 +57,6 @@ __device__ __forceinline__ void upd1(const Params& p, float x, float yo,
   acc.s += dy * dy;
 }
 
-// First pass of a deferred residual reduction, run by the blocks of the
-// next column kernel: block b of nb adds its slice of the per-wavefront
-// partial sums (5 arrays of nparts floats) in double and stores the five
-// results to bsum[5 b + j].  Fixed order, so the sums are deterministic.
-__device__ __forceinline__ void partials_slice(const float* __restrict__ parts,
-                                               int nparts, double* bsum,
-                                               int b, int nb,
-                                               double (*red)[16]) {
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  const int chunk = (nparts + nb - 1) / nb;
-  const int i0 = b * chunk;
-  const int i1 = min(nparts, i0 + chunk);
-  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int i = i0 + tid; i < i1; i += nthr)
-#pragma unroll
-    for (int j = 0; j < 5; ++j) acc[j] += (double)parts[j * nparts + i];
-  const int lane = tid & 63, wv = tid >> 6;
-#pragma unroll
-  for (int j = 0; j < 5; ++j) {
-    double v = acc[j];
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    if (lane == 0) red[j][wv] = v;
-  }
-  __syncthreads();
-  if (tid < 5) {
-    double v = 0.0;
-    for (int w = 0; w < (nthr >> 6); ++w) v += red[tid][w];
-    bsum[5 * b + tid] = v;
-  }
-}
 
 }  // namespace
 
@@ -688,12 +658,12 @@ hipError_t pow2_col(const Geom& g, const Pow2Config& cfg, int nmaps,
 bool solve_regs_fits(int M) { return M >= 1 && M <= 256; }
 
 hipError_t solve_regs(const Geom& g, const Buffers& b, const Params& p,
-                      hipStream_t st) {
+                      hipStream_t st, int ctl_iter, int nbsum) {
   Pow2Config cfg{};
   int G = 1;
   while (G * 16 < g.M) G *= 2;
   cfg.variant = G;
-  return pow2_solve(g, cfg, b, p, st);
+  return pow2_solve(g, cfg, b, p, st, 1, ctl_iter, nbsum);
 }
 
 hipError_t pow2_solve(const Geom& g, const Pow2Config& cfg, const Buffers& b,

@@ -145,6 +145,37 @@ __device__ __forceinline__ void block_sums_store(const UpdateSums& acc,
   }
 }
 
+// First pass of a deferred residual reduction, run by the blocks of the
+// next column kernel: block b of nb adds its slice of the per-wavefront
+// partial sums (5 arrays of nparts floats) in double and stores the five
+// results to bsum[5 b + j].  Fixed order, so the sums are deterministic.
+__device__ __forceinline__ void partials_slice(const float* __restrict__ parts,
+                                               int nparts, double* bsum,
+                                               int b, int nb,
+                                               double (*red)[16]) {
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int chunk = (nparts + nb - 1) / nb;
+  const int i0 = b * chunk;
+  const int i1 = min(nparts, i0 + chunk);
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int i = i0 + tid; i < i1; i += nthr)
+#pragma unroll
+    for (int j = 0; j < 5; ++j) acc[j] += (double)parts[j * nparts + i];
+  const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    double v = acc[j];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if (lane == 0) red[j][wv] = v;
+  }
+  __syncthreads();
+  if (tid < 5) {
+    double v = 0.0;
+    for (int w = 0; w < (nthr >> 6); ++w) v += red[tid][w];
+    bsum[5 * b + tid] = v;
+  }
+}
+
 // Residuals, stopping test and AutoRho from the five global sums
 // (reference host code cbpdn.cu:405-436 and 521-559, with sporco's
 // sqrt(Nc) absolute tolerance term and guarded normalisers).  Runs on one

@@ -130,11 +130,17 @@ hipError_t launch_row_fwd(const Geom& g, int nmaps, const FftPlan& rp,
                           int mode, const float* src, float* Y, float* U,
                           float2* F, const float2* twC, const Ctrl* ctrl,
                           hipStream_t st, int iter = 0);
+// parts / nparts / bsum: first pass of a deferred residual reduction (see
+// launch_solve), as in the power-of-two column kernel.
 hipError_t launch_col(const Geom& g, int nmaps, const FftPlan& cp,
                       float2* F, const float2* twR, int inverse,
-                      const Ctrl* ctrl, hipStream_t st);
+                      const Ctrl* ctrl, hipStream_t st,
+                      const float* parts = nullptr, int nparts = 0,
+                      double* bsum = nullptr);
+// ctl_iter > 0: block 0 first finishes the deferred reduction of that
+// iteration from nbsum block sums (only with the register solve, M <= 256).
 hipError_t launch_solve(const Geom& g, const Buffers& b, const Params& p,
-                        hipStream_t st);
+                        hipStream_t st, int ctl_iter = 0, int nbsum = 0);
 hipError_t launch_row(const Geom& g, const FftPlan& rp, const Buffers& b,
                       const Params& p, int mode, int nblocks_out[1],
                       hipStream_t st);

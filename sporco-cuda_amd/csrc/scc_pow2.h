@@ -46,6 +46,6 @@ hipError_t pow2_row(const Geom& g, const Pow2Config& cfg, const Buffers& b,
 // The register-resident X-solve for any image size (one image, M <= 256).
 bool solve_regs_fits(int M);
 hipError_t solve_regs(const Geom& g, const Buffers& b, const Params& p,
-                      hipStream_t st);
+                      hipStream_t st, int ctl_iter = 0, int nbsum = 0);
 
 }  // namespace scc
