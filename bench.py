@@ -133,8 +133,9 @@ def batched_leg(seed=3, n_images=64, iters=150):
     [128, 768], 64 unit-norm 8x8 filters, per-image lambda log-uniform in
     [0.005, 0.1], rho = 50, 150 fixed iterations.  Times the batched solver
     (second call: autotuning and graph capture happen on the first) against
-    a loop of single-image device solves of the same images, and compares
-    their final residuals."""
+    a loop of single-image device solves of the same images (ratio; their
+    final residuals and maps are compared) and against a loop over the
+    drop-in host API, host arrays in and out (ratio_vs_host_loop)."""
     import numpy as np
     import torch
     import sporco_cuda.cbpdn as cb
@@ -185,6 +186,16 @@ def batched_leg(seed=3, n_images=64, iters=150):
                                  torch.linalg.norm(Y)))
     torch.cuda.synchronize()
     tl = time.perf_counter() - t0
+    # The loop a user of the drop-in API writes: host arrays in and out,
+    # one sporco_cuda.cbpdn.cbpdn call per image (the reference's only
+    # batch form).
+    Dh = np.ascontiguousarray(np.rollaxis(D.cpu().numpy(), 0, 3))
+    Sh = [im.cpu().numpy() for im in imgs]
+    cb.cbpdn(Dh, Sh[0], np.float32(lm[0]), opt)  # workspace of the first shape
+    t0 = time.perf_counter()
+    for i in range(n_images):
+        cb.cbpdn(Dh, Sh[i], np.float32(lm[i]), opt)
+    th = time.perf_counter() - t0
     N = sum(R * C for R, C in sizes)
     return {
         'workload': '%d images, sides multiples of 8 in [128,768] (seed %d), '
@@ -192,6 +203,7 @@ def batched_leg(seed=3, n_images=64, iters=150):
                     '%d iterations' % (n_images, seed, M, K, K, iters),
         'pixels': N,
         'batched_s': tb, 'loop_s': tl, 'ratio': tl / tb,
+        'host_loop_s': th, 'ratio_vs_host_loop': th / tb,
         'batched_Mpix_iters_per_s': N * iters / tb / 1e6,
         'loop_Mpix_iters_per_s': N * iters / tl / 1e6,
         'max_rel_residual_diff': worst, 'max_rel_Y_diff': ydiff,
