@@ -1,6 +1,6 @@
 // Mixed-radix Stockham FFT with a run-time plan, for the generic-size
 // kernels (any image size).  nt transforms of length L live in LDS at
-// A + t * ld, element i at A + t * ld + gpad(i); a stage reads buffer A and
+// A + t * ld, element i at A + t * ld + gpad(i) (= i); a stage reads buffer A and
 // writes buffer B (ping-pong).  Stage s of radix Rx with Ns = product of the
 // earlier radices runs nb = L / Rx butterflies per transform; butterfly j
 // (k = j mod Ns) reads elements j + r nb (r < Rx), multiplies them by the
@@ -29,26 +29,24 @@ using reg::c_add;
 using reg::c_mul;
 using reg::c_sub;
 
-// Padded LDS index: one spare slot after every 16 complex values, so that
-// the strided butterfly writes spread over the banks.
-__host__ __device__ __forceinline__ constexpr int gpad(int i) {
-  return i + (i >> 4);
-}
-// Complex slots of one padded buffer holding elements 0 .. L (inclusive:
-// the half-spectrum rows of even widths have L + 1 entries).
-__host__ __device__ __forceinline__ constexpr int gld(int L) {
-  return L + (L >> 4) + 1;
-}
+// LDS index of element i.  Unpadded: with run-time plans the index
+// arithmetic of a padded layout (one spare slot per 16 values) cost more
+// than the bank conflicts it avoided (measured 5-13 % slower row and 3-11 %
+// slower column transforms on uneven sizes).
+__host__ __device__ __forceinline__ constexpr int gpad(int i) { return i; }
+// Complex slots of one buffer holding elements 0 .. L (inclusive: the
+// half-spectrum rows of even widths have L + 1 entries).
+__host__ __device__ __forceinline__ constexpr int gld(int L) { return L + 1; }
 
 // Image sides whose generic column and row transforms fit the LDS of one
-// block: two padded buffers of a column in 64 KiB, of a row in 60 KiB.
+// block: two buffers of a column in 64 KiB, of a row in 60 KiB.
 __host__ __device__ inline bool sides_fit_lds(int R, int C) {
   const int Lr = (C % 2 == 0) ? C / 2 : C;
   return R >= 1 && C >= 1 && 16 * gld(R) <= 65536 && 16 * gld(Lr) <= 61440;
 }
 
 // Widest power-of-two panel (<= H columns) whose block-wide column
-// transform (two padded buffers per column) fits `budget` bytes of LDS (at
+// transform (two buffers per column) fits `budget` bytes of LDS (at
 // least one column).
 __host__ __device__ inline int col_panel_width(int R, int H, int budget) {
   int cap = budget / (16 * gld(R));

@@ -37,7 +37,7 @@ struct FastDiv {
 
 // --------------------------------------------------------------------------
 // Mixed-radix Stockham FFT over nt transforms of length pl.L held in LDS at
-// A + t*ld (padded element index gen::gpad).  Ping-pongs between A and B and
+// A + t*ld (element index gen::gpad).  Ping-pongs between A and B and
 // returns the buffer holding the result.  DIR = +1 forward
 // (W = exp(-2 pi i / L)), -1 inverse (unnormalised).  fft_block: every
 // thread of the block calls it (block barriers between stages); fft_wave:
@@ -127,7 +127,7 @@ __device__ inline float2* fft_generic(float2* A, float2* B, int nt, int ld,
 // Wavefronts of the wavefront-per-row blocks.
 constexpr int kRowWaves = 4;
 
-// Real value n of a row held as complex pairs (even widths) in a padded
+// Real value n of a row held as complex pairs (even widths) in an LDS
 // buffer.
 __device__ __forceinline__ float& real_at(float2* row, int n) {
   return reinterpret_cast<float*>(row + gen::gpad(n >> 1))[n & 1];
