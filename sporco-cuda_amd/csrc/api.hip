@@ -39,43 +39,13 @@ int set_error(int code, const std::string& msg) {
   return code;
 }
 
-// Radix decomposition for the generic Stockham kernels: 4s, then 2s, then
-// odd factors in increasing order.
-bool make_plan(int L, int tw_stride, FftPlan* pl) {
+// Radix decomposition for the generic Stockham kernels (gen::plan_radices;
+// pfa for the column transforms only).
+bool make_plan(int L, int tw_stride, FftPlan* pl, bool pfa) {
   pl->L = L;
   pl->tw_stride = tw_stride;
-  pl->nstages = 0;
-  int n = L;
-  auto push = [&](int r) {
-    if (pl->nstages >= kMaxStages) return false;
-    pl->radix[pl->nstages++] = r;
-    return true;
-  };
-  // Power-of-two part 2^a: radix-8 stages, the rest as one radix-4 or
-  // radix-2 stage (8 x 2 as 4 x 4); then the odd primes in ascending order.
-  int a = 0;
-  while (n % 2 == 0) {
-    n /= 2;
-    ++a;
-  }
-  const int e8 = a / 3;
-  const int rest = a % 3;
-  for (int i = 0; i < e8 - (rest == 1 && e8 > 0 ? 1 : 0); ++i)
-    if (!push(8)) return false;
-  if (rest == 1 && e8 > 0) {
-    if (!push(4) || !push(4)) return false;
-  } else if (rest == 1) {
-    if (!push(2)) return false;
-  } else if (rest == 2) {
-    if (!push(4)) return false;
-  }
-  for (int p = 3; n > 1; p += 2) {
-    while (n % p == 0) {
-      if (!push(p)) return false;
-      n /= p;
-    }
-  }
-  return true;
+  pl->nstages = gen::plan_radices(L, pl->radix, kMaxStages, pfa);
+  return pl->nstages >= 0;
 }
 
 int make_geom(int R, int C, int M, Geom* g, bool* pow2, Pow2Config* p2) {
@@ -204,8 +174,8 @@ int get_workspace(int device, int R, int C, int M, Workspace** out) {
   ws->device = device;
   make_geom(R, C, M, &ws->g, &ws->pow2, &ws->p2);
   const Geom& g = ws->g;
-  if (!make_plan(g.Lr, g.even ? 2 : 1, &ws->rowp) ||
-      !make_plan(R, 1, &ws->colp))
+  if (!make_plan(g.Lr, g.even ? 2 : 1, &ws->rowp, false) ||
+      !make_plan(R, 1, &ws->colp, true))
     return fail(SCC_ERR_UNSUPPORTED, "FFT length has too many factors");
   const size_t nreal = (size_t)M * g.N;
   const size_t ncplx = (size_t)M * g.Hc;

@@ -535,7 +535,8 @@ int get_bws(int device, int M, const std::vector<int>& sizes, BWs** out) {
     g.N = (long long)g.R * g.C;
     g.Hc = (long long)g.R * g.H;
     g.lgR = -1;
-    if (!make_plan(g.Lr, g.even ? 2 : 1, &im.rp) || !make_plan(g.R, 1, &im.cp))
+    if (!make_plan(g.Lr, g.even ? 2 : 1, &im.rp, false) ||
+        !make_plan(g.R, 1, &im.cp, true))
       return set_error(SCC_ERR_UNSUPPORTED, "FFT length has too many factors");
     im.sqrt_nc = std::sqrt((double)M * (double)g.N);
     oreal[i] = nreal;

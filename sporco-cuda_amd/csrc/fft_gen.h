@@ -151,10 +151,53 @@ __host__ __device__ __forceinline__ void dft_odd(float2* x) {
   x[0] = y0;
 }
 
+// Inverse of a modulo m (a, m coprime, m small).
+__host__ __device__ constexpr int inv_mod(int a, int m) {
+  for (int i = 1; i < m; ++i)
+    if ((a * i) % m == 1) return i;
+  return 1;
+}
+
+// In-place length A B DFT, A in {2, 4}, B an odd prime <= 7, by the prime-
+// factor (Good-Thomas) index maps: with n = (B n1 + A n2) mod AB and k the
+// residue with k = k1 (mod A), k = k2 (mod B), W_AB^{n k} = W_A^{n1 k1}
+// W_B^{n2 k2}, so A B-point DFTs followed by B A-point DFTs need no
+// twiddles between them (all indices are compile-time register names).
+template <int A, int B, int DIR>
+__host__ __device__ __forceinline__ void dft_pfa(float2* x) {
+  constexpr int N = A * B;
+  constexpr int eB = inv_mod(B % A, A), eA = inv_mod(A % B, B);
+  float2 t[A][B];
+#pragma unroll
+  for (int n1 = 0; n1 < A; ++n1)
+#pragma unroll
+    for (int n2 = 0; n2 < B; ++n2) t[n1][n2] = x[(B * n1 + A * n2) % N];
+#pragma unroll
+  for (int n1 = 0; n1 < A; ++n1) dft_odd<B, DIR>(t[n1]);
+#pragma unroll
+  for (int k2 = 0; k2 < B; ++k2) {
+    float2 u[A];
+#pragma unroll
+    for (int n1 = 0; n1 < A; ++n1) u[n1] = t[n1][k2];
+    reg::Dft<A, DIR>::run(u);
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1)
+      x[(B * eB * k1 + A * eA * k2) % N] = u[k1];
+  }
+}
+
 template <int RX, int DIR>
 __host__ __device__ __forceinline__ void bfly(float2* x) {
   if constexpr (RX == 2 || RX == 4 || RX == 8 || RX == 16)
     reg::Dft<RX, DIR>::run(x);
+  else if constexpr (RX == 6)
+    dft_pfa<2, 3, DIR>(x);
+  else if constexpr (RX == 10)
+    dft_pfa<2, 5, DIR>(x);
+  else if constexpr (RX == 12)
+    dft_pfa<4, 3, DIR>(x);
+  else if constexpr (RX == 14)
+    dft_pfa<2, 7, DIR>(x);
   else
     dft_odd<RX, DIR>(x);
 }
@@ -185,7 +228,7 @@ __host__ __device__ __forceinline__ GStage make_stage(int L, int Rx, int Ns) {
   return s;
 }
 
-// Register-butterfly stage (RX <= 13).
+// Register-butterfly stage (RX <= 14).
 template <int RX, int DIR>
 __host__ __device__ __forceinline__ void stage_reg(const float2* A, float2* B,
                                                    int ld, int nt,
@@ -318,12 +361,71 @@ __host__ __device__ __forceinline__ void stage_prime(const float2* A,
   }
 }
 
-// One stage of radix Rx: A -> B.
-template <int DIR>
+// Radix decomposition of a length L >= 1 into at most `max` stages, in
+// order; returns the number of stages (0 for L = 1), or -1 if more are
+// needed.  The power-of-two part 2^a runs as radix-8 stages; with `pfa` a
+// remainder of 2 or 4 merges with the smallest odd prime into one prime-
+// factor stage (6, 10, 12 or 14) when there is one, else it runs as one
+// radix-4 or radix-2 stage (8 x 2 as 4 x 4); the remaining odd primes
+// follow in ascending order.  Fewer, wider stages mean fewer passes over
+// LDS and fewer partly idle item rounds (measured: 768-point columns 12 %
+// faster); the row kernels, whose extra stage code cost more than the
+// merge gained, take plans without them.
+inline int plan_radices(int L, int* radix, int max, bool pfa = true) {
+  int ns = 0;
+  auto push = [&](int r) {
+    if (ns >= max) return false;
+    radix[ns++] = r;
+    return true;
+  };
+  int n = L, a = 0;
+  while (n % 2 == 0) {
+    n /= 2;
+    ++a;
+  }
+  const int e8 = a / 3;
+  const int rest = 1 << (a % 3);
+  int p0 = 0;  // smallest odd prime factor
+  for (int p = 3; p <= n && !p0; p += 2)
+    if (n % p == 0) p0 = p;
+  if (pfa && rest > 1 && p0 > 0 && rest * p0 <= 14) {
+    for (int i = 0; i < e8; ++i)
+      if (!push(8)) return -1;
+    if (!push(rest * p0)) return -1;
+    n /= p0;
+  } else {
+    for (int i = 0; i < e8 - (rest == 2 && e8 > 0 ? 1 : 0); ++i)
+      if (!push(8)) return -1;
+    if (rest == 2 && e8 > 0) {
+      if (!push(4) || !push(4)) return -1;
+    } else if (rest > 1) {
+      if (!push(rest)) return -1;
+    }
+  }
+  for (int p = 3; n > 1; p += 2)
+    while (n % p == 0) {
+      if (!push(p)) return -1;
+      n /= p;
+    }
+  return ns;
+}
+
+// One stage of radix Rx: A -> B.  PFA: also the prime-factor radices 6,
+// 10, 12 and 14 (plans from plan_radices(..., true)).
+template <int DIR, bool PFA = true>
 __host__ __device__ inline void stage(const float2* A, float2* B, int ld,
                                       int nt, const GStage& s,
                                       const float2* tw, int tws, int tid,
                                       int nthr) {
+  if constexpr (PFA) {
+    switch (s.Rx) {
+      case 6: stage_reg<6, DIR>(A, B, ld, nt, s, tw, tws, tid, nthr); return;
+      case 10: stage_reg<10, DIR>(A, B, ld, nt, s, tw, tws, tid, nthr); return;
+      case 12: stage_reg<12, DIR>(A, B, ld, nt, s, tw, tws, tid, nthr); return;
+      case 14: stage_reg<14, DIR>(A, B, ld, nt, s, tw, tws, tid, nthr); return;
+      default: break;
+    }
+  }
   switch (s.Rx) {
     case 2: stage_reg<2, DIR>(A, B, ld, nt, s, tw, tws, tid, nthr); break;
     case 4: stage_reg<4, DIR>(A, B, ld, nt, s, tw, tws, tid, nthr); break;

@@ -50,7 +50,10 @@ __device__ __forceinline__ int uni(int v) {
   return __builtin_amdgcn_readfirstlane(v);
 }
 
-template <int DIR>
+// PFA: the plan may hold the prime-factor radices 6, 10, 12, 14 (column
+// plans; row plans are built without them, which keeps the row kernels'
+// stage code smaller).
+template <int DIR, bool PFA = false>
 __device__ float2* fft_block(float2* A, float2* B, int nt, int ld,
                              const FftPlan& pl, const float2* __restrict__ tw,
                              int tid, int nthr) {
@@ -59,7 +62,7 @@ __device__ float2* fft_block(float2* A, float2* B, int nt, int ld,
   for (int s = 0; s < nst; ++s) {
     const int Rx = uni(pl.radix[s]);
     const gen::GStage st = gen::make_stage(L, Rx, Ns);
-    gen::stage<DIR>(A, B, ld, nt, st, tw, tws, tid, nthr);
+    gen::stage<DIR, PFA>(A, B, ld, nt, st, tw, tws, tid, nthr);
     __syncthreads();
     float2* tmp = A;
     A = B;
@@ -77,7 +80,7 @@ __device__ float2* fft_wave(float2* A, float2* B, const FftPlan& pl,
   for (int s = 0; s < nst; ++s) {
     const int Rx = uni(pl.radix[s]);
     const gen::GStage st = gen::make_stage(L, Rx, Ns);
-    gen::stage<DIR>(A, B, 0, 1, st, tw, tws, lane, 64);
+    gen::stage<DIR, false>(A, B, 0, 1, st, tw, tws, lane, 64);
     reg::wave_sync();
     float2* tmp = A;
     A = B;
@@ -116,12 +119,13 @@ __device__ __forceinline__ Geom geom_uni(const Geom& s) {
   return g;
 }
 
+template <bool PFA = false>
 __device__ inline float2* fft_generic(float2* A, float2* B, int nt, int ld,
                                       const FftPlan& pl,
                                       const float2* __restrict__ tw, float dir,
                                       int tid, int nthr) {
-  return dir > 0.f ? fft_block<1>(A, B, nt, ld, pl, tw, tid, nthr)
-                   : fft_block<-1>(A, B, nt, ld, pl, tw, tid, nthr);
+  return dir > 0.f ? fft_block<1, PFA>(A, B, nt, ld, pl, tw, tid, nthr)
+                   : fft_block<-1, PFA>(A, B, nt, ld, pl, tw, tid, nthr);
 }
 
 // Wavefronts of the wavefront-per-row blocks.
@@ -312,7 +316,7 @@ __device__ inline void col_body(const Geom& g, const FftPlan& cp, float2* F,
     A[c * ld + gen::gpad(r)] = base[i];
   }
   __syncthreads();
-  const float2* Z = fft_generic(A, B, w, ld, cp, twR, dir, tid, nthr);
+  const float2* Z = fft_generic<true>(A, B, w, ld, cp, twR, dir, tid, nthr);
   for (int i = tid; i < n; i += nthr) {
     const int r = dw.div(i);
     const int c = i - r * w;

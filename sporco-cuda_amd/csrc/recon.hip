@@ -132,7 +132,8 @@ int get_rws(int R, int C, int M, int B, RWs** out) {
   w->B = B;
   make_geom(R, C, M, &w->g, &w->pow2, &w->p2);
   const Geom& g = w->g;
-  if (!make_plan(g.Lr, g.even ? 2 : 1, &w->rp) || !make_plan(R, 1, &w->cp))
+  if (!make_plan(g.Lr, g.even ? 2 : 1, &w->rp, false) ||
+      !make_plan(R, 1, &w->cp, true))
     return set_error(SCC_ERR_UNSUPPORTED, "FFT length has too many factors");
   const size_t hc = (size_t)g.Hc;
   int rc;

@@ -176,7 +176,7 @@ hipError_t launch_obj_final(const double* pf, int nf, const double* pl,
 // Host helpers of the driver (api.hip): the thread's last-error message
 // returned by scc_last_error, and the mixed-radix plan of one FFT length.
 int set_error(int code, const std::string& msg);
-bool make_plan(int L, int tw_stride, FftPlan* pl);
+bool make_plan(int L, int tw_stride, FftPlan* pl, bool pfa);
 struct Pow2Config;
 // Geometry of an (R, C) image with M maps; *pow2 tells whether the
 // register-resident power-of-two kernels apply (their config in *p2).

@@ -3,8 +3,9 @@
 // become the end of the item loop), for one transform per wavefront
 // (nt = 1, 64 items at a time) and several per block (nt > 1, 256), and
 // compares the result with a direct double-precision DFT.  Also checks the
-// float-based divider against integer division.  Prints one line per case
-// and exits non-zero on failure.
+// float-based divider against integer division and the plan builder
+// (gen::plan_radices) on every length up to 8192.  Prints one line per
+// case and exits non-zero on failure.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -110,12 +111,53 @@ int main() {
       {264, {8, 3, 11}, 2},     {17, {17}, 1},          {31, {31}, 1},
       {232, {8, 29}, 2},        {712, {8, 89}, 1},      {664, {8, 83}, 2},
       {184, {8, 23}, 1},        {97, {97}, 1},          {200, {8, 5, 5}, 2},
-      {1, {}, 1},
+      {1, {}, 1},               {384, {8, 8, 6}, 2},    {768, {8, 8, 12}, 1},
+      {480, {8, 12, 5}, 1},     {6, {6}, 1},            {10, {10}, 2},
+      {12, {12}, 1},            {14, {14}, 1},          {80, {8, 10}, 2},
+      {112, {8, 14}, 1},        {420, {4, 3, 5, 7}, 1}, {420, {10, 6, 7}, 2},
+      {336, {8, 6, 7}, 1},
   };
   for (const Case& c : cases)
     for (int dir : {1, -1}) {
       bad += check(c.L, c.r, c.tws, dir, 1, 64);
       bad += check(c.L, c.r, c.tws, dir, 3, 256);
     }
+  // The plan builder: every length up to 8192 factors into radices the
+  // stages support, whose product is the length, within 12 stages.
+  // Without pfa (row plans) no radix is an even composite.
+  int badplan = 0;
+  for (int L = 1; L <= 8192; ++L)
+    for (bool pfa : {true, false}) {
+    int r[12];
+    const int ns = plan_radices(L, r, 12, pfa);
+    int prod = 1;
+    bool ok = ns >= 0;
+    for (int i = 0; ok && i < ns; ++i) {
+      const int x = r[i];
+      ok = x == 2 || x == 4 || x == 8 || (x >= 3 && x % 2 == 1) ||
+           (pfa && (x == 6 || x == 10 || x == 12 || x == 14));
+      prod *= x;
+    }
+    if (!ok || prod != L) {
+      if (badplan < 5) printf("plan_radices(%d) wrong\n", L);
+      ++badplan;
+    }
+  }
+  printf("plan_radices %s\n", badplan ? "FAIL" : "ok");
+  bad += badplan ? 1 : 0;
+  // Transforms with the builder's own plans: row plans for every multiple
+  // of 4 in the row lengths of images with sides in [128, 768], column
+  // plans for a spread of column lengths.
+  for (int L = 4; L <= 384; L += 4) {
+    int r[12];
+    const int ns = plan_radices(L, r, 12, false);
+    bad += check(L, std::vector<int>(r, r + ns), 2, 1, 1, 64);
+  }
+  for (int L : {768, 760, 744, 720, 696, 672, 600, 504, 462, 390, 210, 150}) {
+    int r[12];
+    const int ns = plan_radices(L, r, 12, true);
+    bad += check(L, std::vector<int>(r, r + ns), 1, -1, 1, 64);
+    bad += check(L, std::vector<int>(r, r + ns), 1, 1, 2, 256);
+  }
   return bad;
 }

@@ -1,9 +1,10 @@
 """Host-compiled checks of the device FFT code (no GPU): the register-
 resident power-of-two FFT phases (csrc/fft_reg.h) and the run-time-plan
-mixed-radix stages of the generic kernels (csrc/fft_gen.h), each run item by
-item on the CPU and compared with a direct double-precision DFT (relative
-error < 2e-6 / 3e-6), plus the float-based divider against integer
-division.  The programs live in tests/cpp and are compiled with hipcc as
+mixed-radix stages of the generic kernels (csrc/fft_gen.h, including the
+prime-factor radices 6, 10, 12, 14 and the plans its builder makes), each
+run item by item on the CPU and compared with a direct double-precision DFT
+(relative error < 2e-6 / 3e-6), plus the float-based divider against
+integer division.  The programs live in tests/cpp and are compiled with hipcc as
 host code."""
 
 import os
