@@ -268,16 +268,54 @@ __host__ __device__ __forceinline__ void stage_reg(const float2* A, float2* B,
   }
 }
 
-// Prime radix P >= 17, outputs paired as in dft_odd: with the twiddled
+// Prime radix P >= 17 (any odd radix the register butterflies do not
+// cover), in two passes with a barrier between them.
+//
+// Pass 1 (stage_prime_twiddle, only when Ns > 1): the stage twiddles in
+// place, x_n *= W_L^{r e1} for n = j + r nb, e1 = (j mod Ns) tstep, one
+// product per element instead of one per element and work item.
+//
+// Pass 2 (stage_prime): outputs paired as in dft_odd: with the (twiddled)
 // inputs x_r, a_r = x_r + x_{P-r}, b_r = x_r - x_{P-r} (r = 1 .. h,
 // h = (P - 1) / 2) and z = W_P^{q r} (conjugated for the inverse),
 //   t_q = x_0 + sum_r Re(z) a_r,  v_q = sum_r Im(z) b_r,
 //   X_q = t_q + i v_q,  X_{P-q} = t_q - i v_q,  X_0 = x_0 + sum_r a_r.
 // A work item computes the pairs q0 .. q0 + kQB - 1 of butterfly j (the
-// item with q0 = 1 also X_0).  The stage twiddles W_L^{r e1} and the phases
-// advance by one complex product per input pair and are re-read from the
-// table every 16 pairs, which bounds their rounding drift; the twiddle of
-// x_{P-r} is W_L^{P e1} conj(W_L^{r e1}).
+// item with q0 = 1 also X_0).  The phases advance by one complex product
+// per input pair and are re-read from the table every 16 pairs, which
+// bounds their rounding drift.
+// True for the radices stage() sends to stage_prime (those without a
+// register butterfly).
+__host__ __device__ __forceinline__ bool prime_stage(const GStage& s) {
+  const int r = s.Rx;
+  return !(r == 2 || r == 3 || r == 4 || r == 5 || r == 6 || r == 7 ||
+           r == 8 || r == 10 || r == 11 || r == 12 || r == 13 || r == 14);
+}
+
+template <int DIR>
+__host__ __device__ __forceinline__ void stage_prime_twiddle(
+    float2* A, int ld, int nt, const GStage& s, const float2* tw, int tws,
+    int tid, int nthr) {
+  const int L = s.L, nb = s.nb, Ns = s.Ns;
+  const GDiv dL(L), dnb(nb), dns(Ns);
+  const int total = nt * L;
+#pragma unroll 1
+  for (int idx = tid; idx < total; idx += nthr) {
+    int t = 0, n = idx;
+    if (nt > 1) {
+      t = dL.div(idx);
+      n = idx - t * L;
+    }
+    const int r = dnb.div(n);
+    const int j = n - r * nb;
+    const int k = s.p2 ? (j & (Ns - 1)) : j - dns.div(j) * Ns;
+    if (r > 0 && k > 0) {
+      float2* x = A + t * ld + gpad(n);
+      *x = c_mul(*x, tw_at<DIR>(tw, r * k * s.tstep, tws));
+    }
+  }
+}
+
 template <int DIR>
 __host__ __device__ __forceinline__ void stage_prime(const float2* A,
                                                      float2* B, int ld, int nt,
@@ -303,10 +341,6 @@ __host__ __device__ __forceinline__ void stage_prime(const float2* A,
     const int k = s.p2 ? (j & (Ns - 1)) : j - dns.div(j) * Ns;
     const int q0 = 1 + qb * kQB;
     const float2* in = A + t * ld;
-    const int e1 = k * s.tstep;  // stage twiddle exponent step; Rx e1 < L
-    const float2 ws = tw_at<DIR>(tw, e1, tws);
-    const float2 wpe = tw_at<DIR>(tw, Rx * e1, tws);
-    float2 wr = ws;
     const float2 x0 = in[gpad(j)];
     float2 y0 = x0;
     float2 tq[kQB], vq[kQB], z[kQB], stp[kQB];
@@ -321,12 +355,8 @@ __host__ __device__ __forceinline__ void stage_prime(const float2* A,
     }
 #pragma unroll 1
     for (int r = 1; r <= h; ++r) {
-      float2 xa = in[gpad(j + r * nb)];
-      float2 xb = in[gpad(j + (Rx - r) * nb)];
-      if (k) {
-        xa = c_mul(xa, wr);
-        xb = c_mul(xb, c_mul(wpe, make_float2(wr.x, -wr.y)));
-      }
+      const float2 xa = in[gpad(j + r * nb)];
+      const float2 xb = in[gpad(j + (Rx - r) * nb)];
       const float2 a = c_add(xa, xb);
       const float2 b = c_sub(xa, xb);
       y0 = c_add(y0, a);
@@ -338,12 +368,10 @@ __host__ __device__ __forceinline__ void stage_prime(const float2* A,
         vq[q].y += z[q].y * b.y;
       }
       if (((r + 1) & 15) == 0) {
-        wr = tw_at<DIR>(tw, e1 * (r + 1), tws);
 #pragma unroll
         for (int q = 0; q < kQB; ++q)
           z[q] = tw_at<DIR>(tw, ((qq[q] * (r + 1)) % Rx) * nb, tws);
       } else {
-        wr = c_mul(wr, ws);
 #pragma unroll
         for (int q = 0; q < kQB; ++q) z[q] = c_mul(z[q], stp[q]);
       }
@@ -408,6 +436,20 @@ inline int plan_radices(int L, int* radix, int max, bool pfa = true) {
       n /= p;
     }
   return ns;
+}
+
+// Stage pre-pass: the in-place twiddles of a prime stage (prime_stage)
+// with Ns > 1; the caller runs it on every thread, then a barrier, then
+// stage().  No-op for the other stages (stage_pre_needed false).
+__host__ __device__ __forceinline__ bool stage_pre_needed(const GStage& s) {
+  return prime_stage(s) && s.Ns > 1;
+}
+template <int DIR>
+__host__ __device__ inline void stage_pre(float2* A, int ld, int nt,
+                                          const GStage& s, const float2* tw,
+                                          int tws, int tid, int nthr) {
+  if (stage_pre_needed(s))
+    stage_prime_twiddle<DIR>(A, ld, nt, s, tw, tws, tid, nthr);
 }
 
 // One stage of radix Rx: A -> B.  PFA: also the prime-factor radices 6,

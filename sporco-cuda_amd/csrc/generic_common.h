@@ -62,6 +62,10 @@ __device__ float2* fft_block(float2* A, float2* B, int nt, int ld,
   for (int s = 0; s < nst; ++s) {
     const int Rx = uni(pl.radix[s]);
     const gen::GStage st = gen::make_stage(L, Rx, Ns);
+    if (gen::stage_pre_needed(st)) {
+      gen::stage_pre<DIR>(A, ld, nt, st, tw, tws, tid, nthr);
+      __syncthreads();
+    }
     gen::stage<DIR, PFA>(A, B, ld, nt, st, tw, tws, tid, nthr);
     __syncthreads();
     float2* tmp = A;
@@ -80,6 +84,10 @@ __device__ float2* fft_wave(float2* A, float2* B, const FftPlan& pl,
   for (int s = 0; s < nst; ++s) {
     const int Rx = uni(pl.radix[s]);
     const gen::GStage st = gen::make_stage(L, Rx, Ns);
+    if (gen::stage_pre_needed(st)) {
+      gen::stage_pre<DIR>(A, 0, 1, st, tw, tws, lane, 64);
+      reg::wave_sync();
+    }
     gen::stage<DIR, false>(A, B, 0, 1, st, tw, tws, lane, 64);
     reg::wave_sync();
     float2* tmp = A;

@@ -1,6 +1,6 @@
 // CPU check of the run-time-plan Stockham FFT stages (csrc/fft_gen.h): runs
-// gen::stage for every work item in turn (the barriers between stages
-// become the end of the item loop), for one transform per wavefront
+// gen::stage (after gen::stage_pre where a stage has one) for every work
+// item in turn (the barriers become the ends of the item loops), for one transform per wavefront
 // (nt = 1, 64 items at a time) and several per block (nt > 1, 256), and
 // compares the result with a direct double-precision DFT.  Also checks the
 // float-based divider against integer division and the plan builder
@@ -45,6 +45,13 @@ static int check(int L, const std::vector<int>& radices, int tws, int dir,
   int Ns = 1;
   for (int r : radices) {
     const GStage st = make_stage(L, r, Ns);
+    if (stage_pre_needed(st))
+      for (int tid = 0; tid < nthr; ++tid) {
+        if (dir > 0)
+          stage_pre<1>(a, ld, nt, st, tw.data(), tws, tid, nthr);
+        else
+          stage_pre<-1>(a, ld, nt, st, tw.data(), tws, tid, nthr);
+      }
     for (int tid = 0; tid < nthr; ++tid) {
       if (dir > 0)
         stage<1>(a, b, ld, nt, st, tw.data(), tws, tid, nthr);
