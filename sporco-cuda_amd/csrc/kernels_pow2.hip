@@ -177,7 +177,7 @@ __global__ __launch_bounds__(256) void k_solve_pow2(Geom g, Buffers b,
     const int m = gi * MPT + i;
     if (valid && m < M) {
       const long long o = (long long)m * g.Hc + f;
-      const float2 d = b.Df[o];
+      const float2 d = load_once(b.Df + o);
       const float2 a = Fi[o];
       const float2 x = cadd(cmul(cconj(d), sh), cscale(a, rho));
       float inv = irho;
