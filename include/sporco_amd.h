@@ -151,7 +151,7 @@ int scc_last_stage_times(double *ms_total, int *launches, int n);
  * iterations with fixed rho, relaxation alpha, scalar lambda, L1 weight 1.
  * All arrays are device pointers: D (M, K, K), S (B, R, C), Y (B, M, R, C),
  * ckpt of scc_unrolled_ckpt_floats(...) floats ((Y, U) every ckpt_every
- * iterations, and the tape of the last two segments: X-hat and the
+ * iterations, and the tape of the last three segments: X-hat and the
  * soft-threshold support of each of their iterations); ckpt may be NULL in
  * the forward for an inference-only solve.  The backward takes dL/dY and
  * writes dL/dD (M, K, K), dL/dS (B, R, C) and dL/dlambda (one float, host

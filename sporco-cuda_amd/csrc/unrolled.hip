@@ -4,7 +4,7 @@
 // Forward: the solver's own kernels over B images of one size sharing the
 // dictionary (column and row kernels run over B*M maps, the solve takes the
 // image index), fixed rho, no stopping test; (Y, U) is saved every k
-// iterations, and the tape of the last two segments is kept with the
+// iterations, and the tape of the last three segments is kept with the
 // checkpoints.  The tape of an iteration is X-hat after the solve (written
 // by the solve kernel) and the soft-threshold support of V = Xr + U as 4
 // bits per coefficient pair (|V| > t and sign V, written by the row
@@ -721,7 +721,7 @@ int fwd_iter(UWs* w, const Params& p, float2* xtape, unsigned* tbits,
 // of the last kTapeSegs segments (X-hat of each of their iterations, then
 // the support bits), which the backward pass reads instead of recomputing
 // them.  Memory stays O(segments x state + kTapeSegs x k x tape).
-constexpr int kTapeSegs = 2;
+constexpr int kTapeSegs = 3;
 
 struct CkptLayout {
   int nseg;

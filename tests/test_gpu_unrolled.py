@@ -117,10 +117,11 @@ def test_unrolled_forward_and_gradients(gpu):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize('ck', [1, 4, 5])
+@pytest.mark.parametrize('ck', [1, 2, 3, 5])
 def test_checkpoint_interval_invariance(gpu, ck):
-    """Recomputed segments (all but the last two) and taped ones give the
-    gradients of a single taped segment."""
+    """Recomputed segments (all but the last three) and taped ones give the
+    gradients of a single taped segment (12 iterations: 9, 3, 1 and 0
+    recomputed segments; 5 ends with a partial segment)."""
     D, S, G = _inputs(3)
     print('checkpoint interval', ck, flush=True)
     a = _gpu_grads(D, S, G, ck)
