@@ -23,10 +23,9 @@ __device__ __forceinline__ float2 cscale(float2 a, float s) {
   return make_float2(a.x * s, a.y * s);
 }
 // Non-temporal load of a value that is read once per pass and not again
-// before much other traffic (the filter spectra in the X-solve): it does not
-// displace the work spectrum, which the next kernel then reads back from the
-// last-level cache (measured: X-solve 161 -> 142 us, inverse columns 94 ->
-// 89 us at 1024 x 1024 x 64).
+// before much other traffic (the filter spectra in the X-solve of large
+// problems): it does not displace data that the next kernel reads back from
+// the last-level cache.
 __device__ __forceinline__ float2 load_once(const float2* p) {
   return __builtin_bit_cast(
       float2, __builtin_nontemporal_load(
