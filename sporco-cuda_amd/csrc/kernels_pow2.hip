@@ -118,10 +118,11 @@ __global__ __launch_bounds__(TC*(R / kColE)) void k_col_pow2(
 // not an AutoRho step, so rho is unchanged for the other blocks.
 // NTD: the filter spectra are read with non-temporal loads (see
 // pow2_solve).
-template <int G, int MPT, bool GRAD, bool NTD>
-__global__ __launch_bounds__(256) void k_solve_pow2(Geom g, Buffers b,
-                                                    Params p, int ctl_iter,
-                                                    int nbsum) {
+// Deferred reduction of iteration ctl_iter (see k_solve_pow2), run by
+// block (0, 0) of a 256-thread solve launch.
+__device__ __forceinline__ void solve_prologue(const Buffers& b,
+                                               const Params& p, int ctl_iter,
+                                               int nbsum) {
   if (ctl_iter > 0 && blockIdx.x == 0 && blockIdx.y == 0 && !b.ctrl->done) {
     __shared__ double red[5][4];
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -145,6 +146,13 @@ __global__ __launch_bounds__(256) void k_solve_pow2(Geom g, Buffers b,
     }
     __syncthreads();
   }
+}
+
+template <int G, int MPT, bool GRAD, bool NTD>
+__global__ __launch_bounds__(256) void k_solve_pow2(Geom g, Buffers b,
+                                                    Params p, int ctl_iter,
+                                                    int nbsum) {
+  solve_prologue(b, p, ctl_iter, nbsum);
   if (b.ctrl->done) return;
   constexpr int FPW = 64 / G;
   const int lane = threadIdx.x & 63;
@@ -203,6 +211,98 @@ __global__ __launch_bounds__(256) void k_solve_pow2(Geom g, Buffers b,
 #pragma unroll
   for (int i = 0; i < MPT; ++i) {
     const int m = gi * MPT + i;
+    if (valid && m < M) {
+      const float inv = GRAD ? id[i] : irho;
+      const float2 x = cscale(csub(bb[i], cmul(cconj(dd[i]), cb)), inv);
+      Fi[(long long)m * g.Hc + f] = x;
+      if (Xt != nullptr) Xt[(long long)m * g.Hc + f] = x;
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// The same solve for 64 < M <= 256 with the maps split over the block's 4
+// wavefronts instead of over more lanes: a block takes 16 consecutive
+// frequencies, wavefront w the maps (4 w + gi) MPT + i of lane group gi, so
+// every load reads 16 consecutive frequencies (a whole 128-byte line) of a
+// map with MPT maps per lane in registers.  The 4 wavefronts' partial sums
+// meet in LDS and are added in a fixed order.
+// --------------------------------------------------------------------------
+template <int MPT, bool GRAD, bool NTD>
+__global__ __launch_bounds__(256) void k_solve_split(Geom g, Buffers b,
+                                                     Params p, int ctl_iter,
+                                                     int nbsum) {
+  __shared__ float part[4][16][3];
+  __shared__ int sdone;
+  solve_prologue(b, p, ctl_iter, nbsum);
+  // The stop flag is read once per block: block 0 may set it while this
+  // block runs, and every thread of the block must reach the barrier below.
+  if (threadIdx.x == 0) sdone = b.ctrl->done;
+  __syncthreads();
+  if (sdone) return;
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int gi = lane >> 4;
+  const int fl = lane & 15;
+  const long long f = (long long)blockIdx.x * 16 + fl;
+  const bool valid = f < g.Hc;
+  const float rho = b.ctrl->rho;
+  const float irho = 1.f / rho;
+  const int M = g.M;
+  float2* const Fi = b.F + (long long)blockIdx.y * M * g.Hc;
+  float2* const Xt =
+      b.xtape ? b.xtape + (long long)blockIdx.y * M * g.Hc : nullptr;
+  float2 sh = make_float2(0.f, 0.f);
+  float gh = 0.f;
+  if (valid) {
+    sh = b.Sf[(long long)blockIdx.y * g.Hc + f];
+    if (GRAD) gh = p.mu * b.ghg[f];
+  }
+  const int m0 = (wv * 4 + gi) * MPT;
+  float2 bb[MPT], dd[MPT];
+  float id[GRAD ? MPT : 1];
+  float2 num = make_float2(0.f, 0.f);
+  float den = 0.f;
+#pragma unroll
+  for (int i = 0; i < MPT; ++i) {
+    const int m = m0 + i;
+    if (valid && m < M) {
+      const long long o = (long long)m * g.Hc + f;
+      const float2 d = NTD ? load_once(b.Df + o) : b.Df[o];
+      const float2 a = Fi[o];
+      const float2 x = cadd(cmul(cconj(d), sh), cscale(a, rho));
+      float inv = irho;
+      if (GRAD) {
+        inv = 1.f / ((p.gw_mode ? b.gw[m] : p.gw_scalar) * gh + rho);
+        id[i] = inv;
+      }
+      num = cadd(num, cscale(cmul(d, x), inv));
+      den += (d.x * d.x + d.y * d.y) * inv;
+      bb[i] = x;
+      dd[i] = d;
+    }
+  }
+#pragma unroll
+  for (int o = 16; o < 64; o <<= 1) {
+    num.x += __shfl_xor(num.x, o, 64);
+    num.y += __shfl_xor(num.y, o, 64);
+    den += __shfl_xor(den, o, 64);
+  }
+  if (gi == 0) {
+    part[wv][fl][0] = num.x;
+    part[wv][fl][1] = num.y;
+    part[wv][fl][2] = den;
+  }
+  __syncthreads();
+  num = make_float2(((part[0][fl][0] + part[1][fl][0]) + part[2][fl][0]) +
+                        part[3][fl][0],
+                    ((part[0][fl][1] + part[1][fl][1]) + part[2][fl][1]) +
+                        part[3][fl][1]);
+  den = ((part[0][fl][2] + part[1][fl][2]) + part[2][fl][2]) + part[3][fl][2];
+  const float2 cb = cscale(num, 1.f / (1.f + den));
+#pragma unroll
+  for (int i = 0; i < MPT; ++i) {
+    const int m = m0 + i;
     if (valid && m < M) {
       const float inv = GRAD ? id[i] : irho;
       const float2 x = cscale(csub(bb[i], cmul(cconj(dd[i]), cb)), inv);
@@ -693,6 +793,23 @@ hipError_t pow2_solve(const Geom& g, const Pow2Config& cfg, const Buffers& b,
       hipLaunchKernelGGL((k_solve_pow2<gg, 16, false, NTD>), dim3(nb),      \
                          dim3(256), 0, st, g, b, p, ctl_iter, nbsum);      \
     return hipGetLastError();                                              \
+  }
+  // 64 < M <= 256: maps split over the block's wavefronts (whole-line
+  // reads), 8 or 16 maps per lane.
+  if (G == 8 || G == 16) {
+    const dim3 nbs((unsigned)((g.Hc + 15) / 16), (unsigned)nimg);
+#define S(MP, NTD)                                                          \
+  if (G == MP && ntd == NTD) {                                             \
+    if (p.grad)                                                            \
+      hipLaunchKernelGGL((k_solve_split<MP, true, NTD>), dim3(nbs),         \
+                         dim3(256), 0, st, g, b, p, ctl_iter, nbsum);      \
+    else                                                                   \
+      hipLaunchKernelGGL((k_solve_split<MP, false, NTD>), dim3(nbs),        \
+                         dim3(256), 0, st, g, b, p, ctl_iter, nbsum);      \
+    return hipGetLastError();                                              \
+  }
+    S(8, false) S(8, true) S(16, false) S(16, true)
+#undef S
   }
   X(1, false) X(2, false) X(4, false) X(8, false) X(16, false)
   X(1, true) X(2, true) X(4, true) X(8, true) X(16, true)

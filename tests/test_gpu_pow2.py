@@ -3,7 +3,8 @@ FP64 oracle run on the same seeded inputs.
 
 The golden fixtures are small odd/even sizes that take the generic kernels;
 these cases take the register-resident kernels (row lengths 128..1024
-complex, column lengths 256..2048, 1..16 lanes per frequency in the solve)
+complex, column lengths 256..2048, 1..4 lanes per frequency in the solve,
+and 65..256 maps split over a block's wavefronts)
 through the same drop-in API.  Tolerance as in test_gpu_parity: relative L2
 error <= max(1e-4, 2 e32) with e32 the FP32 oracle's own error, and the
 iteration count of a stopping rule within one of the oracle's.
@@ -38,6 +39,14 @@ CASES = {
     'autorho_stop_256': (256, 256, 8, 8, 'cbpdn', 0.05, None, 300,
                          {'AutoRho': {'Enabled': True},
                           'RelStopTol': 2e-3}, None),
+    # More than 64 maps: the solve splits the maps over the wavefronts of a
+    # block (8 maps per lane for 65..128, 16 for 129..256).
+    'M100_mask_grad_autorho_stop': (256, 256, 5, 99, 'cbpdngrdmsk', 0.05,
+                                    0.2, 40,
+                                    {'AutoRho': {'Enabled': True,
+                                                 'Period': 3},
+                                     'RelStopTol': 5e-3}, 'vec'),
+    'M160_256': (256, 256, 4, 160, 'cbpdn', 0.05, None, 12, {}, None),
 }
 
 
