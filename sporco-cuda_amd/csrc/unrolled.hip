@@ -476,11 +476,164 @@ hipError_t adj_row2(const Geom& gbm, const Pow2Config& cfg, float2* F,
   return hipErrorNotSupported;
 }
 
+// The adjoint solve with the maps split over the block's 4 wavefronts
+// (used for 32 < M <= 128, where k_adj_solve needs 8 or more lanes per
+// frequency and its reads cover half a 128-byte line of a map): a block
+// takes 16 consecutive frequencies, wavefront w the maps (4 w + gi) MPT + i
+// of lane group gi, so every read covers a whole line.  The per-frequency
+// sums (sigma, then per image num, then sDB and sDX) meet in LDS and are
+// added in a fixed order; every thread passes every barrier.
+template <int MPT>
+__global__ __launch_bounds__(256) void k_adj_solve_split(
+    Geom g, int nimg, float2* F, const float2* __restrict__ Xh,
+    const float2* __restrict__ Df, const float2* __restrict__ Sf,
+    float2* Dhb, float2* Shb, float rho, float invN) {
+  __shared__ float rn[4][16][2];  // sigma, then num per image
+  __shared__ float rs[4][16][4];  // sDB, sDX per image
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int gi = lane >> 4;
+  const int fl = lane & 15;
+  const long long f = (long long)blockIdx.x * 16 + fl;
+  const bool valid = f < g.Hc;
+  const int M = g.M;
+  const int m0 = (wv * 4 + gi) * MPT;
+  const float irho = 1.f / rho;
+  const float wt = valid ? col_mult(g, panel_col(g, f)) : 1.f;
+  const float sin_ = wt * invN;
+  const float sout = rho / wt;
+  float2 dd[MPT], dacc[MPT];
+  float sig = 0.f;
+#pragma unroll
+  for (int i = 0; i < MPT; ++i) {
+    const int m = m0 + i;
+    dacc[i] = make_float2(0.f, 0.f);
+    dd[i] = make_float2(0.f, 0.f);
+    if (valid && m < M) {
+      dd[i] = Df[(long long)m * g.Hc + f];
+      sig += dd[i].x * dd[i].x + dd[i].y * dd[i].y;
+    }
+  }
+  sig += __shfl_xor(sig, 16, 64);
+  sig += __shfl_xor(sig, 32, 64);
+  if (gi == 0) rn[wv][fl][0] = sig;
+  __syncthreads();
+  sig = ((rn[0][fl][0] + rn[1][fl][0]) + rn[2][fl][0]) + rn[3][fl][0];
+  const float isig = 1.f / (rho + sig);
+  __syncthreads();  // sigma read before the first num is written
+  const int i0 = blockIdx.y * kImgGroup;
+  const int i1 = min(nimg, i0 + kImgGroup);
+  for (int img = i0; img < i1; ++img) {
+    float2* Fi = F + (long long)img * M * g.Hc;
+    const float2* Xi = Xh + (long long)img * M * g.Hc;
+    float2 xb[MPT], xh[MPT];
+    float2 sh = make_float2(0.f, 0.f), sb = make_float2(0.f, 0.f);
+    if (valid) {
+      sh = Sf[(long long)img * g.Hc + f];
+      if (wv == 0 && gi == 0) sb = Shb[(long long)img * g.Hc + f];
+    }
+#pragma unroll
+    for (int i = 0; i < MPT; ++i) {
+      const int m = m0 + i;
+      xb[i] = make_float2(0.f, 0.f);
+      xh[i] = make_float2(0.f, 0.f);
+      if (valid && m < M) {
+        const long long o = (long long)m * g.Hc + f;
+        xb[i] = Fi[o];
+        xh[i] = Xi[o];
+      }
+    }
+    float2 num = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < MPT; ++i) {
+      xb[i] = cscale(xb[i], sin_);
+      num = cadd(num, cmul(dd[i], xb[i]));
+    }
+#pragma unroll
+    for (int o = 16; o < 64; o <<= 1) {
+      num.x += __shfl_xor(num.x, o, 64);
+      num.y += __shfl_xor(num.y, o, 64);
+    }
+    if (gi == 0) {
+      rn[wv][fl][0] = num.x;
+      rn[wv][fl][1] = num.y;
+    }
+    __syncthreads();
+    num = make_float2(
+        ((rn[0][fl][0] + rn[1][fl][0]) + rn[2][fl][0]) + rn[3][fl][0],
+        ((rn[0][fl][1] + rn[1][fl][1]) + rn[2][fl][1]) + rn[3][fl][1]);
+    const float2 cb = cscale(num, isig);
+    float2 sDB = make_float2(0.f, 0.f), sDX = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < MPT; ++i) {
+      xb[i] = cscale(csub(xb[i], cmul(cconj(dd[i]), cb)), irho);  // Bbar
+      sDB = cadd(sDB, cmul(dd[i], xb[i]));
+      sDX = cadd(sDX, cmul(dd[i], xh[i]));
+    }
+#pragma unroll
+    for (int o = 16; o < 64; o <<= 1) {
+      sDB.x += __shfl_xor(sDB.x, o, 64);
+      sDB.y += __shfl_xor(sDB.y, o, 64);
+      sDX.x += __shfl_xor(sDX.x, o, 64);
+      sDX.y += __shfl_xor(sDX.y, o, 64);
+    }
+    if (gi == 0) {
+      rs[wv][fl][0] = sDB.x;
+      rs[wv][fl][1] = sDB.y;
+      rs[wv][fl][2] = sDX.x;
+      rs[wv][fl][3] = sDX.y;
+    }
+    // Also orders this image's num reads before the next image's writes.
+    __syncthreads();
+    sDB = make_float2(
+        ((rs[0][fl][0] + rs[1][fl][0]) + rs[2][fl][0]) + rs[3][fl][0],
+        ((rs[0][fl][1] + rs[1][fl][1]) + rs[2][fl][1]) + rs[3][fl][1]);
+    sDX = make_float2(
+        ((rs[0][fl][2] + rs[1][fl][2]) + rs[2][fl][2]) + rs[3][fl][2],
+        ((rs[0][fl][3] + rs[1][fl][3]) + rs[2][fl][3]) + rs[3][fl][3]);
+#pragma unroll
+    for (int i = 0; i < MPT; ++i) {
+      const int m = m0 + i;
+      if (valid && m < M) {
+        const float2 cbb = cconj(xb[i]);
+        dacc[i] = cadd(dacc[i],
+                       csub(cmul(cbb, csub(sh, sDX)), cmul(cconj(xh[i]), sDB)));
+        Fi[(long long)m * g.Hc + f] = cscale(xb[i], sout);
+      }
+    }
+    if (valid && wv == 0 && gi == 0)
+      Shb[(long long)img * g.Hc + f] = cadd(sb, sDB);
+    // The sDB / sDX reads above finish before the next image writes rs:
+    // the next image's num barrier lies between them.
+  }
+  float2* Dg = Dhb + (long long)blockIdx.y * M * g.Hc;
+#pragma unroll
+  for (int i = 0; i < MPT; ++i) {
+    const int m = m0 + i;
+    if (valid && m < M) {
+      float2* d = Dg + (long long)m * g.Hc + f;
+      *d = cadd(*d, dacc[i]);
+    }
+  }
+}
+
 int img_groups(int nimg) { return (nimg + kImgGroup - 1) / kImgGroup; }
 
 hipError_t adj_solve(const Geom& g, int nimg, float2* F, const float2* Xh,
                      const float2* Df, const float2* Sf, float2* Dhb,
                      float2* Shb, float rho, hipStream_t st) {
+  const float invN0 = 1.f / (float)g.N;
+  if (g.M > 32 && g.M <= 128) {
+    const dim3 nbs((unsigned)((g.Hc + 15) / 16), (unsigned)img_groups(nimg));
+#define S(MP)                                                               \
+  if (g.M <= 16 * MP) {                                                    \
+    hipLaunchKernelGGL((k_adj_solve_split<MP>), nbs, dim3(256), 0, st, g,   \
+                       nimg, F, Xh, Df, Sf, Dhb, Shb, rho, invN0);         \
+    return hipGetLastError();                                              \
+  }
+    S(4) S(8)
+#undef S
+  }
   int G = 1;
   while (G * 8 < g.M) G *= 2;
   const long long fpb = 4LL * (64 / G);
