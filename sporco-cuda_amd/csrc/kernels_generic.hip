@@ -97,7 +97,11 @@ __global__ __launch_bounds__(256) void k_col_generic(
   const int lin = blockIdx.y * gridDim.x + blockIdx.x;
   if (nparts > 0 && lin < nred)
     partials_slice(parts, nparts, bsum, lin, nred, red);
-  col_body(g, cp, F, twR, dir, blockIdx.x, blockIdx.y, lds);
+  // Forward: maps last to first, so that the row kernel's most recent
+  // output is read while it is still in the last-level cache.
+  const int m = dir > 0.f ? (int)gridDim.y - 1 - (int)blockIdx.y
+                          : (int)blockIdx.y;
+  col_body(g, cp, F, twR, dir, blockIdx.x, m, lds);
 }
 
 // --------------------------------------------------------------------------

@@ -87,7 +87,12 @@ __global__ __launch_bounds__(TC*(R / kColE)) void k_col_pow2(
   if (nparts > 0 && lin < nred)
     partials_slice(parts, nparts, bsum, lin, nred, red);
   const int p = blockIdx.x;
-  const long long m = blockIdx.y;
+  // The forward transform takes the maps last to first: the row kernel
+  // before it writes them first to last, so its most recent output, still
+  // in the last-level cache, is read before this kernel's own traffic
+  // evicts it (105.7 -> 95.7 us at 1024 x 1024 x 64).
+  const long long m =
+      DIR > 0 ? (long long)gridDim.y - 1 - blockIdx.y : blockIdx.y;
   const int w = (p == g.P - 1) ? g.wlast : TC;
   const int c = threadIdx.x % TC;
   const int t = threadIdx.x / TC;
