@@ -31,12 +31,19 @@
 
 namespace scc {
 namespace {
-thread_local std::string g_err;
+// Message of the last failure on this thread: a fixed buffer, so that
+// reporting an error allocates nothing.
+thread_local char g_err[512];
 }  // namespace
 
-int set_error(int code, const std::string& msg) {
-  g_err = msg;
+int set_error(int code, const char* what, const char* detail) {
+  std::snprintf(g_err, sizeof(g_err), "%s%s%s", what ? what : "",
+                detail ? ": " : "", detail ? detail : "");
   return code;
+}
+
+int set_error(int code, const std::string& msg) {
+  return set_error(code, msg.c_str(), nullptr);
 }
 
 // Radix decomposition for the generic Stockham kernels (gen::plan_radices;
@@ -92,28 +99,36 @@ thread_local bool g_stage_on = false;
 thread_local double g_stage_ms[kNumStages];
 thread_local int g_stage_n[kNumStages];
 
-int fail(int code, const std::string& msg) { return set_error(code, msg); }
+int fail(int code, const char* what, const char* detail = nullptr) {
+  return set_error(code, what, detail);
+}
 
-#define SCC_HIP(call)                                                  \
-  do {                                                                 \
-    hipError_t e_ = (call);                                            \
-    if (e_ != hipSuccess) {                                            \
-      return fail(e_ == hipErrorOutOfMemory ? SCC_ERR_NOMEM           \
-                                            : SCC_ERR_HIP,            \
-                  std::string(#call) + ": " + hipGetErrorString(e_)); \
-    }                                                                  \
+#define SCC_HIP(call)                                                   \
+  do {                                                                  \
+    hipError_t e_ = (call);                                             \
+    if (e_ != hipSuccess) {                                             \
+      return fail(e_ == hipErrorOutOfMemory ? SCC_ERR_NOMEM : SCC_ERR_HIP, \
+                  #call, hipGetErrorString(e_));                        \
+    }                                                                   \
   } while (0)
 
+// A device buffer: owned (allocated by ensure, freed with the workspace),
+// or a view into memory the library does not own (owned = false).
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool owned = true;
   ~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (p && owned) (void)hipFree(p);
   }
 };
 
+// Makes b hold at least `bytes`.  A view that is too small is an error: a
+// view is never replaced by an allocation.
 int ensure(DevBuf& b, size_t bytes) {
-  if (b.bytes >= bytes && b.p) return SCC_OK;
+  if (b.bytes >= bytes && (b.p || bytes == 0)) return SCC_OK;
+  if (!b.owned)
+    return fail(SCC_ERR_UNSUPPORTED, "internal: device buffer view too small");
   if (b.p) {
     (void)hipFree(b.p);
     b.p = nullptr;
@@ -121,17 +136,22 @@ int ensure(DevBuf& b, size_t bytes) {
   }
   if (bytes == 0) return SCC_OK;
   hipError_t e = hipMalloc(&b.p, bytes);
-  if (e != hipSuccess)
-    return fail(SCC_ERR_NOMEM, std::string("hipMalloc of ") +
-                                   std::to_string(bytes) + " bytes: " +
-                                   hipGetErrorString(e));
+  if (e != hipSuccess) {
+    char what[64];
+    std::snprintf(what, sizeof(what), "hipMalloc of %zu bytes", bytes);
+    return fail(SCC_ERR_NOMEM, what, hipGetErrorString(e));
+  }
   b.bytes = bytes;
   return SCC_OK;
 }
 
 // Everything that depends only on (device, R, C, M): buffers, plans, tables.
+// The entry points cache one per device; one whose buffers are views into
+// memory the library does not own has owned = false (and no stream, events
+// or pinned memory of its own).
 struct Workspace {
   int device = -1;
+  bool owned = true;
   Geom g{};
   FftPlan rowp{}, colp{};
   bool pow2 = false;
@@ -139,12 +159,14 @@ struct Workspace {
   DevBuf Y, U, F, Df, Sf, ghg, partials, blocksum, ctrl, hist, twR, twC;
   DevBuf sD, sS, sL1, sGW, sMask;  // staging for the host-pointer entry
   DevBuf oG, oZ, oE, oPart, oVal, oHist;  // objective evaluation
-  DevBuf flag;                            // negative-weight test
+  DevBuf flag;                            // negative-weight / mask tests
   bool ghg_ready = false;
   hipStream_t stream = nullptr;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-  Ctrl* hctrl = nullptr;  // pinned host copy of the control block
+  Ctrl* hctrl = nullptr;  // host copy of the control block (pinned if owned)
+  std::mutex mu;          // held for the whole of a solve in this workspace
   ~Workspace() {
+    if (!owned) return;
     if (hctrl) (void)hipHostFree(hctrl);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
@@ -152,14 +174,33 @@ struct Workspace {
   }
 };
 
+// g_mu guards the cache only; a solve holds its workspace's own lock, so
+// solves in different workspaces (devices, shapes) run concurrently.  An
+// evicted workspace lives on until the solves holding it return.
 std::mutex g_mu;
-std::map<std::tuple<int, int, int, int>, std::unique_ptr<Workspace>> g_ws;
+std::map<std::tuple<int, int, int, int>, std::shared_ptr<Workspace>> g_ws;
 
-int get_workspace(int device, int R, int C, int M, Workspace** out) {
+// Geometry and FFT plans of (R, C, M) into ws.
+int init_geometry(Workspace* ws, int R, int C, int M) {
+  make_geom(R, C, M, &ws->g, &ws->pow2, &ws->p2);
+  if (!make_plan(ws->g.Lr, ws->g.even ? 2 : 1, &ws->rowp, false) ||
+      !make_plan(R, 1, &ws->colp, true))
+    return fail(SCC_ERR_UNSUPPORTED, "FFT length has too many factors");
+  return SCC_OK;
+}
+
+// Number of row-kernel blocks (residual partial sums) of the geometry.
+int ws_row_blocks(const Workspace* ws) {
+  return ws->pow2 ? pow2_row_blocks(ws->g, ws->p2) : row_blocks(ws->g);
+}
+
+int get_workspace(int device, int R, int C, int M,
+                  std::shared_ptr<Workspace>* out) {
+  std::lock_guard<std::mutex> lk(g_mu);
   auto key = std::make_tuple(device, R, C, M);
   auto it = g_ws.find(key);
   if (it != g_ws.end()) {
-    *out = it->second.get();
+    *out = it->second;
     return SCC_OK;
   }
   // Keep at most one workspace per device: shapes of consecutive calls
@@ -170,66 +211,47 @@ int get_workspace(int device, int R, int C, int M, Workspace** out) {
     else
       ++i;
   }
-  auto ws = std::make_unique<Workspace>();
+  auto ws = std::make_shared<Workspace>();
   ws->device = device;
-  make_geom(R, C, M, &ws->g, &ws->pow2, &ws->p2);
+  int rc;
+  if ((rc = init_geometry(ws.get(), R, C, M))) return rc;
   const Geom& g = ws->g;
-  if (!make_plan(g.Lr, g.even ? 2 : 1, &ws->rowp, false) ||
-      !make_plan(R, 1, &ws->colp, true))
-    return fail(SCC_ERR_UNSUPPORTED, "FFT length has too many factors");
   const size_t nreal = (size_t)M * g.N;
   const size_t ncplx = (size_t)M * g.Hc;
-  int rc;
-  if ((rc = ensure(ws->Y, nreal * 4)) || (rc = ensure(ws->U, nreal * 4)) ||
-      (rc = ensure(ws->F, ncplx * 8)) || (rc = ensure(ws->Df, ncplx * 8)) ||
+  // U is allocated by the first solve that needs it (the (Y, U) state or
+  // AutoRho), not here.
+  if ((rc = ensure(ws->Y, nreal * 4)) || (rc = ensure(ws->F, ncplx * 8)) ||
+      (rc = ensure(ws->Df, ncplx * 8)) ||
       (rc = ensure(ws->Sf, (size_t)g.Hc * 8)) ||
       (rc = ensure(ws->ctrl, sizeof(Ctrl))) ||
       (rc = ensure(ws->twR, (size_t)R * 8)) ||
       (rc = ensure(ws->twC, (size_t)C * 8)))
     return rc;
-  const int nb = ws->pow2 ? pow2_row_blocks(g, ws->p2) : row_blocks(g);
   // Block sums of the first reduction pass (at most 64 blocks, in the
   // reduction kernel or in the power-of-two column kernel).
-  if ((rc = ensure(ws->partials, (size_t)5 * nb * 4)) ||
+  if ((rc = ensure(ws->partials, (size_t)5 * ws_row_blocks(ws.get()) * 4)) ||
       (rc = ensure(ws->blocksum, (size_t)5 * 64 * 8)))
     return rc;
   SCC_HIP(hipStreamCreateWithFlags(&ws->stream, hipStreamNonBlocking));
   for (auto& e : ws->ev) SCC_HIP(hipEventCreate(&e));
   SCC_HIP(hipHostMalloc((void**)&ws->hctrl, sizeof(Ctrl)));
-  // Twiddle tables W_L^t = exp(-2 pi i t / L), computed in double.
-  std::vector<float2> tw;
-  for (int pass = 0; pass < 2; ++pass) {
-    const int L = pass == 0 ? R : C;
-    tw.resize(L);
-    for (int t = 0; t < L; ++t) {
-      const double a = -2.0 * M_PI * (double)t / (double)L;
-      tw[t] = make_float2((float)std::cos(a), (float)std::sin(a));
-    }
-    SCC_HIP(hipMemcpy(pass == 0 ? ws->twR.p : ws->twC.p, tw.data(),
-                      (size_t)L * 8, hipMemcpyHostToDevice));
-  }
-  *out = ws.get();
+  SCC_HIP(launch_twiddles((float2*)ws->twR.p, R, ws->stream));
+  SCC_HIP(launch_twiddles((float2*)ws->twC.p, C, ws->stream));
+  SCC_HIP(hipStreamSynchronize(ws->stream));
+  *out = ws;
   g_ws[key] = std::move(ws);
   return SCC_OK;
 }
 
-int ensure_ghg(Workspace* ws) {
+// GradReg spectrum |F[-1 1]|^2 along rows plus along columns, the closed
+// form of the spectra the reference computes by FFT (cbpdn_grd.cu:295-314),
+// on the solve stream.
+int ensure_ghg(Workspace* ws, hipStream_t st) {
   if (ws->ghg_ready) return SCC_OK;
-  const Geom& g = ws->g;
-  int rc = ensure(ws->ghg, (size_t)g.Hc * 4);
+  int rc = ensure(ws->ghg, (size_t)ws->g.Hc * 4);
   if (rc) return rc;
-  // |F[-1 1]|^2 along rows plus along columns, the closed form of the
-  // spectra the reference computes by FFT (cbpdn_grd.cu:295-314).
-  std::vector<float> h((size_t)g.Hc);
-  for (int r = 0; r < g.R; ++r)
-    for (int c = 0; c < g.H; ++c) {
-      const double v = 4.0 - 2.0 * std::cos(2.0 * M_PI * r / g.R) -
-                       2.0 * std::cos(2.0 * M_PI * c / g.C);
-      h[panel_off(g, r, c)] = (float)v;
-    }
-  SCC_HIP(hipMemcpy(ws->ghg.p, h.data(), h.size() * 4,
-                    hipMemcpyHostToDevice));
-  ws->ghg_ready = true;
+  SCC_HIP(launch_ghg(ws->g, (float*)ws->ghg.p, st));
+  ws->ghg_ready = ws->owned;  // a view is rebuilt on every call
   return SCC_OK;
 }
 
@@ -436,7 +458,9 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
   p.l1_mode = rs.l1_mode;
   p.grad = grad ? 1 : 0;
   p.gw_mode = rs.gw_mode;
-  p.track = g_hist_on ? 1 : 0;
+  // A workspace of views records no history or objective (their buffers
+  // would need allocation).
+  p.track = (g_hist_on && ws->owned) ? 1 : 0;
   p.l1_scalar = 1.f;
   p.gw_scalar = 1.f;
   if (rs.l1_mode == 0)
@@ -445,12 +469,12 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
     SCC_HIP(hipMemcpyAsync(&p.gw_scalar, gw, 4, hipMemcpyDeviceToHost, st));
   SCC_HIP(hipStreamSynchronize(st));
   int rc;
-  if (grad && (rc = ensure_ghg(ws))) return rc;
+  if (grad && (rc = ensure_ghg(ws, st))) return rc;
   const int maxit = o.MaxMainIter;
   if (p.track && (rc = ensure(ws->hist, (size_t)3 * std::max(maxit, 1) * 4)))
     return rc;
   // Objective evaluation after every iteration (Verbose, or requested).
-  const bool obj_on = g_obj_on || o.Verbose == 1;
+  const bool obj_on = (g_obj_on && ws->owned) || o.Verbose == 1;
   const bool aux = o.AuxVarObj == 1;
   // The state is V (one real array per coefficient) unless the objective
   // of every iteration needs Y, or a threshold is negative (shrink(0) != 0:
@@ -466,6 +490,20 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
     thr_ok = neg == 0;
   }
   p.vstate = (!obj_on && thr_ok) ? 1 : 0;
+  // With AutoRho the V state decodes the AMS map after a rho update as
+  // (1 - W) (Y + U / rsf), which is (Y, U / rsf) only where W is 0 or 1:
+  // another mask value keeps the (Y, U) state.
+  if (p.vstate && p.autorho && rs.mask) {
+    if ((rc = ensure(ws->flag, 4))) return rc;
+    int bad = 0;
+    SCC_HIP(launch_any_nonbinary(mask, g.N, (int*)ws->flag.p, st));
+    SCC_HIP(hipMemcpyAsync(&bad, ws->flag.p, 4, hipMemcpyDeviceToHost, st));
+    SCC_HIP(hipStreamSynchronize(st));
+    if (bad) p.vstate = 0;
+  }
+  // U is read only by the (Y, U) state and written by AutoRho steps.
+  const bool need_u = !p.vstate || p.autorho;
+  if (need_u && (rc = ensure(ws->U, (size_t)M * g.N * 4))) return rc;
   if (obj_on &&
       ((rc = ensure(ws->oG, (size_t)M * g.Hc * 8)) ||
        (rc = ensure(ws->oZ, aux ? 4 : (size_t)M * g.N * 4)) ||
@@ -478,7 +516,7 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
 
   Buffers b{};
   b.Y = (float*)ws->Y.p;
-  b.U = (float*)ws->U.p;
+  b.U = need_u ? (float*)ws->U.p : nullptr;
   b.F = (float2*)ws->F.p;
   b.Df = (float2*)ws->Df.p;
   b.Sf = (float2*)ws->Sf.p;
@@ -495,7 +533,7 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
 
   const size_t nreal = (size_t)M * g.N;
   const size_t ncplx = (size_t)M * g.Hc;
-  SCC_HIP(hipEventRecord(ws->ev[0], st));
+  if (ws->ev[0]) SCC_HIP(hipEventRecord(ws->ev[0], st));
   // Setup (cbpdn.cu:276-298): spectra of the padded filters and the image.
   SCC_HIP(launch_pad(D, b.Y, M, o.DICT_ROW_SIZE, o.DICT_COL_SIZE, g.R, g.C,
                      st));
@@ -516,7 +554,7 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
   }
   // Y = U = 0 and A-hat = 0 for the first iteration (cbpdn.cu:321-326).
   SCC_HIP(hipMemsetAsync(b.Y, 0, nreal * 4, st));
-  SCC_HIP(hipMemsetAsync(b.U, 0, nreal * 4, st));
+  if (need_u) SCC_HIP(hipMemsetAsync(b.U, 0, nreal * 4, st));
   SCC_HIP(hipMemsetAsync(b.F, 0, ncplx * 8, st));
   Ctrl c0{};
   c0.rho = o.rho;
@@ -524,7 +562,7 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
   *ws->hctrl = c0;
   SCC_HIP(hipMemcpyAsync(b.ctrl, ws->hctrl, sizeof(Ctrl),
                          hipMemcpyHostToDevice, st));
-  SCC_HIP(hipEventRecord(ws->ev[1], st));
+  if (ws->ev[1]) SCC_HIP(hipEventRecord(ws->ev[1], st));
 
   // Stage timing: one event before the first stage of an iteration and one
   // after every stage.
@@ -534,8 +572,9 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
     g_stage_ms[k] = 0.0;
     g_stage_n[k] = 0;
   }
+  const bool stage_on = g_stage_on && ws->owned;
   auto mark = [&](int stage) -> hipError_t {
-    if (!g_stage_on) return hipSuccess;
+    if (!stage_on) return hipSuccess;
     hipEvent_t e;
     hipError_t err = hipEventCreate(&e);
     if (err != hipSuccess) return err;
@@ -643,7 +682,7 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
       if (ws->hctrl->done) break;
     }
   }
-  SCC_HIP(hipEventRecord(ws->ev[2], st));
+  if (ws->ev[2]) SCC_HIP(hipEventRecord(ws->ev[2], st));
   if (p.vstate)  // Y = shrink(V), into the output or in place
     SCC_HIP(launch_state_to_y(g, b, p, Yout != nullptr ? Yout : b.Y, st));
   else if (Yout != nullptr)
@@ -667,9 +706,11 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
   g_info.epri = cf.epri;
   g_info.edua = cf.edua;
   float ms = 0.f;
-  if (hipEventElapsedTime(&ms, ws->ev[0], ws->ev[1]) == hipSuccess)
+  if (ws->ev[0] &&
+      hipEventElapsedTime(&ms, ws->ev[0], ws->ev[1]) == hipSuccess)
     g_info.setup_ms = ms;
-  if (hipEventElapsedTime(&ms, ws->ev[1], ws->ev[2]) == hipSuccess)
+  if (ws->ev[1] &&
+      hipEventElapsedTime(&ms, ws->ev[1], ws->ev[2]) == hipSuccess)
     g_info.solve_ms = ms;
   if (obj_on) {
     g_obj.resize((size_t)4 * cf.iter);
@@ -699,11 +740,12 @@ int solve_host(const float* D, const float* S, float lambda, float mu,
   if (rc) return rc;
   const scc_algopt& o = rs.o;
   if ((rc = check_device(o.device))) return rc;
-  std::lock_guard<std::mutex> lk(g_mu);
   DeviceGuard dg(o.device);
-  Workspace* ws;
+  std::shared_ptr<Workspace> wsp;
   const int R = o.IMG_ROW_SIZE, C = o.IMG_COL_SIZE, M = o.DICT_M_SIZE;
-  if ((rc = get_workspace(o.device, R, C, M, &ws))) return rc;
+  if ((rc = get_workspace(o.device, R, C, M, &wsp))) return rc;
+  Workspace* ws = wsp.get();
+  std::lock_guard<std::mutex> lk(ws->mu);
   const size_t nd = (size_t)M * o.DICT_ROW_SIZE * o.DICT_COL_SIZE;
   const size_t nl1 = rs.n_l1;
   const size_t ngw = grad ? (size_t)std::max(o.WEIGHT_SIZE, 1) : 0;
@@ -812,12 +854,13 @@ int scc_cbpdn_device(const float* D, const float* S, float lambda, float mu,
   if (rc) return rc;
   const scc_algopt& o = rs.o;
   if ((rc = check_device(o.device))) return rc;
-  std::lock_guard<std::mutex> lk(g_mu);
   DeviceGuard dg(o.device);
-  Workspace* ws;
+  std::shared_ptr<Workspace> wsp;
   if ((rc = get_workspace(o.device, o.IMG_ROW_SIZE, o.IMG_COL_SIZE,
-                          o.DICT_M_SIZE, &ws)))
+                          o.DICT_M_SIZE, &wsp)))
     return rc;
+  Workspace* ws = wsp.get();
+  std::lock_guard<std::mutex> lk(ws->mu);
   hipStream_t st = stream ? (hipStream_t)stream : ws->stream;
   return solve_core(ws, rs, D, S, lambda, mu, grad, o.L1Weight, o.GrdWeight,
                     o.Weight, Y, st);
@@ -829,7 +872,7 @@ int scc_last_info(scc_info* info) {
   return SCC_OK;
 }
 
-const char* scc_last_error(void) { return g_err.c_str(); }
+const char* scc_last_error(void) { return g_err; }
 
 void scc_set_history(int enabled) { g_hist_on = enabled != 0; }
 

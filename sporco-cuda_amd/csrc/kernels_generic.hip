@@ -15,6 +15,7 @@
 //   k_reduce          residual norms, stopping test, AutoRho (device side)
 
 #include <algorithm>
+#include <cmath>
 
 #include "generic_common.h"
 #include "scc_pow2.h"
@@ -340,6 +341,54 @@ __global__ __launch_bounds__(256) void k_any_negative(const float* w,
 }
 
 __global__ void k_clear_flag(int* flag) { flag[0] = 0; }
+
+__global__ __launch_bounds__(256) void k_any_nonbinary(const int* mask,
+                                                       long long n,
+                                                       int* flag) {
+  int bad = 0;
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n;
+       i += (long long)gridDim.x * 256)
+    bad |= (unsigned)mask[i] > 1u;
+  if (__any(bad) && (threadIdx.x & 63) == 0) flag[0] = 1;
+}
+
+hipError_t launch_any_nonbinary(const int* mask, long long n, int* flag,
+                                hipStream_t st) {
+  hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, st, flag);
+  const unsigned nb = (unsigned)std::min<long long>((n + 255) / 256, 2048);
+  hipLaunchKernelGGL(k_any_nonbinary, dim3(nb), dim3(256), 0, st, mask, n,
+                     flag);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_twiddles(float2* tw, int L) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= L) return;
+  const double a = -2.0 * M_PI * (double)t / (double)L;
+  tw[t] = make_float2((float)cos(a), (float)sin(a));
+}
+
+hipError_t launch_twiddles(float2* tw, int L, hipStream_t st) {
+  hipLaunchKernelGGL(k_twiddles, dim3((L + 255) / 256), dim3(256), 0, st, tw,
+                     L);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_ghg(Geom g, float* ghg) {
+  const long long f = blockIdx.x * 256LL + threadIdx.x;
+  if (f >= g.Hc) return;
+  const int r = (int)(f / g.H);
+  const int c = (int)(f - (long long)r * g.H);
+  const double v = 4.0 - 2.0 * cos(2.0 * M_PI * r / g.R) -
+                   2.0 * cos(2.0 * M_PI * c / g.C);
+  ghg[panel_off(g, r, c)] = (float)v;
+}
+
+hipError_t launch_ghg(const Geom& g, float* ghg, hipStream_t st) {
+  hipLaunchKernelGGL(k_ghg, dim3((unsigned)((g.Hc + 255) / 256)), dim3(256),
+                     0, st, g, ghg);
+  return hipGetLastError();
+}
 
 hipError_t launch_any_negative(const float* w, long long n, int* flag,
                                hipStream_t st) {

@@ -150,6 +150,16 @@ int row_blocks(const Geom& g);
 // flag[0] = 1 if any of the n device values is negative, else 0.
 hipError_t launch_any_negative(const float* w, long long n, int* flag,
                                hipStream_t st);
+// flag[0] = 1 if any of the n mask entries is neither 0 nor 1, else 0.
+hipError_t launch_any_nonbinary(const int* mask, long long n, int* flag,
+                                hipStream_t st);
+// Twiddle table tw[t] = exp(-2 pi i t / L), t < L, from double-precision
+// cos / sin rounded to float.
+hipError_t launch_twiddles(float2* tw, int L, hipStream_t st);
+// Gradient-regularisation spectrum 4 - 2 cos(2 pi r / R) - 2 cos(2 pi c / C)
+// (double precision, rounded to float) of every half-spectrum frequency, in
+// the panel layout of g.
+hipError_t launch_ghg(const Geom& g, float* ghg, hipStream_t st);
 // Y = shrink(V) of a state array holding V (Params::vstate) into out.
 hipError_t launch_state_to_y(const Geom& g, const Buffers& b,
                              const Params& p, float* out, hipStream_t st);
@@ -176,6 +186,8 @@ hipError_t launch_obj_final(const double* pf, int nf, const double* pl,
 // Host helpers of the driver (api.hip): the thread's last-error message
 // returned by scc_last_error, and the mixed-radix plan of one FFT length.
 int set_error(int code, const std::string& msg);
+// The same without building a string: "what: detail" (detail may be NULL).
+int set_error(int code, const char* what, const char* detail);
 bool make_plan(int L, int tw_stride, FftPlan* pl, bool pfa);
 struct Pow2Config;
 // Geometry of an (R, C) image with M maps; *pow2 tells whether the

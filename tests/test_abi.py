@@ -258,3 +258,4 @@ def test_one_hip_runtime_whatever_the_import_order():
             'if "libamdhip64" in l}))' % os.path.join(ROOT, 'sporco-cuda_amd'))
     out = subprocess.check_output([sys.executable, '-c', code], text=True)
     assert out.strip().splitlines()[-1] == '1', out
+

@@ -77,6 +77,9 @@ def test_batched_equals_single_and_oracle(gpu, autotune, branches):
                            rho=np.float32(3.0), max_iter=40, rel_tol=0.0,
                            dtype=np.float32)
         Yg = np.rollaxis(X, 2, 0)
+        print('batched %dx%d: vs single %.3e, vs FP64 oracle %.3e, FP32 '
+              'oracle %.3e' % (S.shape + (rel(X, Xs), rel(Yg, Yo),
+                                          rel(Y32, Yo))))
         assert rel(Yg, Yo) <= max(1e-4, 2 * rel(Y32, Yo))
 
 

@@ -103,6 +103,8 @@ def test_golden(gpu, path):
     # non-convex, and after ~20 iterations FP32 iterates that agree to
     # 3e-7 can take locally different paths (two builds differing only in
     # FFT rounding end 4.7e-4 apart).  Its bounds are 1e-3 and MSE 1e-6.
+    print('%s: rel. L2 error %.3e, FP32 oracle %.3e, MSE %.3e'
+          % (os.path.basename(path)[:-4], err, e32, mse))
     sensitive = os.path.basename(path).startswith('ref04')
     tol_mse = 1e-6 if sensitive else 1e-8
     tol_rel = 1e-3 if sensitive else 1e-4
@@ -156,3 +158,26 @@ def test_device_queries(gpu):
     assert f >= 0 and t > 0
     nm = cu.device_name()
     assert nm is not None and nm != ''
+
+
+@pytest.mark.gpu
+def test_nonbinary_mask_autorho(gpu):
+    """A mask holding values other than 0 and 1 with AutoRho: the one-array
+    state cannot decode the AMS map after a rho update there, so the solver
+    must keep the (Y, U) state and match the FP64 oracle, whose map-0 update
+    is (1 - W) V for any W (cbpdn_kernels.cu:1137-1155)."""
+    import sporco_cuda.cbpdn as cb
+    from oracle import cbpdn as ocb
+    c = load_case([p for p in FIXTURES if 'ref11' in p][0])
+    W = np.random.default_rng(7).integers(0, 3, c['W'].shape)
+    W = W.astype(np.float32)
+    opt = dict(c['opt'], MaxMainIter=12)
+    opt['AutoRho'] = dict(opt['AutoRho'], Enabled=True, Period=2)
+    X = cb.cbpdnmsk(c['D'], c['S'], W, c['lmbda'], opt)
+    Xo = ocb.cbpdnmsk(c['D'], c['S'], W.astype(np.int32), c['lmbda'], opt)
+    X32 = ocb.cbpdnmsk(c['D'], c['S'], W.astype(np.int32), c['lmbda'], opt,
+                       dtype=np.float32)
+    err, e32 = rel_err(X, Xo), rel_err(X32, Xo)
+    print('non-binary mask + AutoRho: rel. error %.3e (FP32 oracle %.3e)'
+          % (err, e32))
+    assert err <= max(1e-4, 10 * e32), (err, e32)

@@ -112,4 +112,5 @@ def test_pow2_parity(gpu, name):
         Xo = np.asarray(run_api(ocb, c, opt))
     e32 = rel_err(run_api(_Oracle32(), c, opt), Xo)
     err = rel_err(X, Xo)
+    print('%s: rel. L2 error %.3e, FP32 oracle %.3e' % (name, err, e32))
     assert err <= max(1e-4, 2 * e32), (err, e32, info)
