@@ -48,8 +48,14 @@ def diff_matrices(R, C):
 
 
 def admm(D, S, lmbda, rho, n_iter, relax_param=1.8, l1_weight=1.0, mu=None,
-         grad_weight=1.0, mask=None):
+         grad_weight=1.0, mask=None, nonneg=False, nobndry=False):
     """Run exactly n_iter dense ADMM iterations.
+
+    nonneg / nobndry: sporco's NonNegCoef (coefficients clamped at 0 after
+    the soft threshold) and NoBndryCross (coefficients at positions p = (i,
+    j) whose filter support [i - Kh + 1, i] x [j - Kw + 1, j] wraps around
+    the image's last row or column, i > R - Kh or j > C - Kw, held at 0),
+    both on the non-impulse maps.
 
     Returns a list of (X, Y, U) triples, one per iteration, each (M, R, C).
     """
@@ -83,6 +89,13 @@ def admm(D, S, lmbda, rho, n_iter, relax_param=1.8, l1_weight=1.0, mu=None,
         t = (lmbda / rho) * w[m0:]
         a = np.abs(v[m0:]) - t
         yn[m0:] = np.where(a > 0, a, 0.0) * np.where(v[m0:] < 0, -1.0, 1.0)
+        if nonneg:
+            yn[m0:] = np.where(yn[m0:] > 0, yn[m0:], 0.0)
+        if nobndry:
+            Kh, Kw = D.shape[1:]
+            p = np.arange(N)
+            wrap = ((p // C) > R - Kh) | ((p % C) > C - Kw)
+            yn[m0:, wrap] = 0.0
         if mask is not None:
             yn[0] = (1.0 - np.asarray(mask, np.float64).reshape(-1)) * v[0]
         u = v - yn

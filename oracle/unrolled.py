@@ -181,3 +181,98 @@ def central_difference(f, theta_plus, theta_minus, eps0=1e-3, min_eps=1e-7):
             return (lp - lm) / (2 * eps), eps
         eps /= 10
     raise RuntimeError('no kink-free step found')
+
+
+def forward_tape_torch(D, S, lmbda, rho, n_iter, alpha=1.8, supports=None,
+                       tape=True):
+    """``forward_numpy`` on torch tensors (float64 on S's device): the same
+    loop, the same optional fixed supports (a list of boolean tensors, one
+    (B, M, R, C) per iteration), and the tape (Xh, V) of every iteration
+    when ``tape``.  Returns (Y, state dict with Dh, Sh, sigma, tape)."""
+    import torch
+    dt = torch.float64
+    dev = S.device
+    D = D.to(dt)
+    S = S.to(dt)
+    M, K, _ = D.shape
+    B, R, C = S.shape
+    Dp = torch.zeros((M, R, C), dtype=dt, device=dev)
+    Dp[:, :K, :K] = D
+    Dh = torch.fft.rfft2(Dp)
+    del Dp
+    Sh = torch.fft.rfft2(S)
+    sigma = torch.sum(Dh.real ** 2 + Dh.imag ** 2, 0)
+    Y = torch.zeros((B, M, R, C), dtype=dt, device=dev)
+    U = torch.zeros_like(Y)
+    thr = float(lmbda) / float(rho)
+    tp = []
+    for it in range(n_iter):
+        Ah = torch.fft.rfft2(Y - U)
+        Bm = torch.conj(Dh)[None] * Sh[:, None] + rho * Ah
+        del Ah
+        cb = torch.sum(Dh[None] * Bm, 1) / (rho + sigma)
+        Xh = (Bm - torch.conj(Dh)[None] * cb[:, None]) / rho
+        del Bm
+        X = torch.fft.irfft2(Xh, s=(R, C))
+        V = alpha * X + (1 - alpha) * Y + U
+        del X
+        if supports is None:
+            Y = torch.sign(V) * torch.clamp(V.abs() - thr, min=0)
+        else:
+            Y = torch.where(supports[it], V - thr * torch.sign(V),
+                            torch.zeros((), dtype=dt, device=dev))
+        U = V - Y
+        if tape:
+            tp.append((Xh, V))
+        del Xh, V
+    return Y, dict(Dh=Dh, Sh=Sh, sigma=sigma, tape=tp)
+
+
+def adjoint_torch(D, S, lmbda, rho, n_iter, Ybar, alpha=1.8, Dhbar0=None,
+                  supports=None):
+    """``adjoint_numpy`` on torch tensors (float64 on S's device), stage by
+    stage in the same order.  Returns (dbar, sbar, lmbda_bar)."""
+    import torch
+    dt = torch.float64
+    dev = S.device
+    M, K, _ = D.shape
+    B, R, C = S.shape
+    N = R * C
+    H = C // 2 + 1
+    w = torch.as_tensor(col_weights(C, H), dtype=dt, device=dev)
+    _, st = forward_tape_torch(D, S, lmbda, rho, n_iter, alpha, supports)
+    Dh, Sh, sigma = st['Dh'], st['Sh'], st['sigma']
+    thr = float(lmbda) / float(rho)
+    Yb = Ybar.to(dt).clone()
+    Ub = torch.zeros_like(Yb)
+    Dhb = torch.zeros_like(Dh) if Dhbar0 is None else Dhbar0.clone()
+    Shb = torch.zeros_like(Sh)
+    lb = 0.0
+    for it in range(n_iter - 1, -1, -1):
+        Xh, V = st['tape'][it]
+        mask = V.abs() > thr if supports is None else supports[it]
+        Vb = Ub + (Yb - Ub) * mask
+        lb += float(torch.sum((Ub - Yb) * torch.sign(V) * mask)) / rho
+        del Yb, Ub
+        Xb = alpha * Vb
+        Yb = (1 - alpha) * Vb
+        Ub = Vb
+        Xhb = w * torch.fft.rfft2(Xb) / N
+        del Xb
+        cb = torch.sum(Dh[None] * Xhb, 1) / (rho + sigma)
+        Bb = (Xhb - torch.conj(Dh)[None] * cb[:, None]) / rho
+        del Xhb
+        sDB = torch.sum(Dh[None] * Bb, 1)
+        sDX = torch.sum(Dh[None] * Xh, 1)
+        Dhb += torch.sum(-torch.conj(Xh) * sDB[:, None]
+                         - torch.conj(Bb) * sDX[:, None]
+                         + torch.conj(Bb) * Sh[:, None], 0)
+        Shb += sDB
+        Ab = N * torch.fft.irfft2(rho * Bb / w, s=(R, C))
+        del Bb
+        Yb = Yb + Ab
+        Ub = Ub - Ab
+        st['tape'][it] = None
+    dbar = (N * torch.fft.irfft2(Dhb / w, s=(R, C)))[:, :K, :K]
+    sbar = N * torch.fft.irfft2(Shb / w, s=(R, C))
+    return dbar, sbar, lb

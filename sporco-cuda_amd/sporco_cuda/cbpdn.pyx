@@ -102,24 +102,8 @@ def default_options(opt=None, grad=False):
     """Complete option dict: sporco's ConvBPDN (``grad=False``) or
     ConvBPDNGradReg (``grad=True``) defaults, updated from ``opt``
     (nested ``AutoRho`` entries are merged)."""
-    d = {
-        'Verbose': False, 'MaxMainIter': 1000, 'AbsStopTol': 0.0,
-        'RelStopTol': 1e-3, 'RelaxParam': 1.8, 'rho': None,
-        'AutoRho': {'Enabled': True, 'Period': 1, 'AutoScaling': True,
-                    'Scaling': 1000.0, 'RsdlRatio': 1.2, 'RsdlTarget': None,
-                    'StdResiduals': False},
-        'AuxVarObj': False, 'HighMemSolve': False, 'NonNegCoef': False,
-        'NoBndryCross': False, 'L1Weight': 1.0,
-    }
-    if grad:
-        d['GradWeight'] = 1.0
-    if opt is not None:
-        for k in opt:
-            if k == 'AutoRho':
-                d['AutoRho'].update(dict(opt['AutoRho']))
-            else:
-                d[k] = opt[k]
-    return d
+    from sporco_cuda.options import default_options as _d
+    return _d(opt, grad)
 
 
 def last_info():

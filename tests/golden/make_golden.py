@@ -180,8 +180,8 @@ def save(name, api, D, S, W, lmbda, mu, opt):
     np.savez_compressed(
         os.path.join(HERE, name + '.npz'), D=D, S=S,
         W=(W if W is not None else np.zeros((0, 0), np.float32)),
-        X=X.astype(np.float64), lmbda=np.float32(lmbda),
-        mu=np.float32(mu if mu is not None else 0.0),
+        X=X.astype(np.float64), lmbda=np.float64(lmbda),
+        mu=np.float64(mu if mu is not None else 0.0),
         api=np.array(api), opt_json=np.array(json.dumps(scal)),
         iterations=np.int64(info['iterations']), **arrs)
     return X, info

@@ -112,6 +112,12 @@ DENSE_CASES = {
     'grad_array': dict(mu=0.2, mask=False, w='array', relax=1.5),
     'mask': dict(mu=None, mask=True, w='vector', relax=1.8),
     'grad_mask': dict(mu=0.1, mask=True, w='scalar', relax=1.8),
+    'nonneg': dict(mu=None, mask=False, w='vector', relax=1.8,
+                   nonneg=True),
+    'nobndry': dict(mu=None, mask=False, w='scalar', relax=1.8,
+                    nobndry=True),
+    'grad_mask_nonneg_nobndry': dict(mu=0.2, mask=True, w='array',
+                                     relax=1.6, nonneg=True, nobndry=True),
 }
 
 
@@ -134,13 +140,20 @@ def test_iterates_match_dense_admm(name):
         w[0] = 0
     gw = np.array([0.5, 1.0, 2.0])
     its = []
+    nn, nb = cse.get('nonneg', False), cse.get('nobndry', False)
     Y, info = ocb.solve(D, S, 0.1, rho=1.3, max_iter=12, rel_tol=0.0,
                         relax_param=cse['relax'], l1_weight=w, mu=cse['mu'],
-                        grad_weight=gw, mask=mask,
+                        grad_weight=gw, mask=mask, nonneg=nn, nobndry=nb,
                         callback=lambda i, X, Y, U: its.append(
                             (X.copy(), Y.copy(), U.copy())))
     ref = dense.admm(D, S, 0.1, 1.3, 12, relax_param=cse['relax'],
-                     l1_weight=w, mu=cse['mu'], grad_weight=gw, mask=mask)
+                     l1_weight=w, mu=cse['mu'], grad_weight=gw, mask=mask,
+                     nonneg=nn, nobndry=nb)
+    if nn:
+        assert np.all(ref[-1][1][1 if cse['mask'] else 0:] >= 0)
+        assert np.any(ref[-1][0] < -1e-3)  # the clamp is active
+    if nb:
+        assert np.all(ref[-1][1][1 if cse['mask'] else 0:, R - K + 1:] == 0)
     for (X, Y, U), (Xd, Yd, Ud) in zip(its, ref):
         assert np.allclose(X, Xd, atol=1e-10)
         assert np.allclose(Y, Yd, atol=1e-10)
@@ -353,3 +366,66 @@ def test_golden_fixtures_reproduce():
         assert info['iterations'] == int(z['iterations'])
         assert str(z['api']) == api
         json.loads(str(z['opt_json']))
+
+
+def scalar_autorho(sv, lm, rho, alpha, n, period, tau, mu, xi, autoscaling):
+    """AutoRho on a closed-form problem: an impulse dictionary (one 1 x 1
+    filter of value 1) on a constant image s makes every coefficient run
+    the same scalar recursion
+        x = (s + rho (y - u)) / (1 + rho),  v = alpha x + (1 - alpha) y + u,
+        y = sign(v) max(|v| - lambda / rho, 0),  u = v - y,
+    with normalised residuals r = |x - y| / max(|x|, |y|) and
+    s = rho |y_prev - y| / (rho |u|).  The penalty update is the
+    residual-balancing rule of cbpdn.cu:521-559 (sporco's update_rho):
+    after iteration i > 1 with i % period == 0, rho is multiplied by
+    rsf = rhomlt when r > xi mu s, by 1 / rhomlt when s > (mu / xi) r (else
+    1), rhomlt = min(tau, sqrt(q) if q >= 1 else sqrt(1 / q)) with
+    q = r / (xi s) under AutoScaling, tau otherwise, and u is divided by
+    rsf.  Returns the rho used by every iteration."""
+    import math
+    y = u = 0.0
+    out = []
+    for i in range(1, n + 1):
+        yp = y
+        x = (sv + rho * (y - u)) / (1.0 + rho)
+        v = alpha * x + (1.0 - alpha) * y + u
+        y = math.copysign(max(abs(v) - lm / rho, 0.0), v)
+        u = v - y
+        r = abs(x - y) / max(abs(x), abs(y))
+        s = abs(yp - y) / abs(u)
+        out.append(rho)
+        if i > 1 and i % period == 0:
+            q = r / (xi * s)
+            rm = min(tau, math.sqrt(q if q >= 1 else 1.0 / q)) \
+                if autoscaling else tau
+            rsf = rm if r > xi * mu * s else (1.0 / rm if s > (mu / xi) * r
+                                              else 1.0)
+            rho *= rsf
+            u /= rsf
+    return out
+
+
+AUTORHO_SCALAR = [
+    # (s, lambda, rho, alpha, iterations, period, tau, mu, xi, autoscaling):
+    # rho rises, then falls (autoscaled factors); alternates by tau; rises
+    # then falls every iteration (autoscaled).
+    (-0.8, 0.2, 0.1, 1.5, 12, 2, 10.0, 1.2, 2.0, True),
+    (1.0, 0.05, 1.0, 1.8, 12, 1, 3.0, 1.2, 2.0, False),
+    (2.0, 0.5, 0.1, 1.5, 12, 1, 3.0, 1.2, 2.0, True),
+]
+
+
+@pytest.mark.parametrize('case', AUTORHO_SCALAR)
+def test_autorho_sequence_on_scalar_problem(case):
+    sv, lm, rho, alpha, n, period, tau, mu, xi, asc = case
+    expect = scalar_autorho(sv, lm, rho, alpha, n, period, tau, mu, xi, asc)
+    assert len(set(np.round(expect, 12))) >= 3  # rho changes at least twice
+    S = np.full((4, 5), sv)
+    D = np.ones((1, 1, 1))
+    _, info = ocb.solve(D, S, lm, rho=rho, max_iter=n, rel_tol=0.0,
+                        relax_param=alpha, track=True,
+                        autorho=dict(Period=period, AutoScaling=asc,
+                                     Scaling=tau, RsdlRatio=mu,
+                                     RsdlTarget=xi))
+    got = [h[2] for h in info['history']]
+    assert np.allclose(got, expect, rtol=1e-9, atol=0)

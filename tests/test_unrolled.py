@@ -106,3 +106,28 @@ def test_forced_support_branch_gradient():
         assert abs(fd - an) <= 1e-6 * max(1.0, abs(an)), (fd, an)
     fd = (f(D, S, lm + eps) - f(D, S, lm - eps)) / (2 * eps)
     assert abs(fd - l) <= 1e-8 * max(1.0, abs(l)), (fd, l)
+
+
+@pytest.mark.parametrize('fixed', [False, True])
+def test_torch_tape_adjoint_equals_numpy(fixed):
+    """adjoint_torch / forward_tape_torch (the FP64 checker of the GPU
+    gradients at full size) restate adjoint_numpy / forward_numpy, also
+    with fixed supports (some flipped)."""
+    D, S, lm, rho, n, G = _problem(4)
+    sup = None
+    if fixed:
+        _, st = ur.forward_numpy(D, S, lm, rho, n)
+        rng = np.random.default_rng(9)
+        sup = [(np.abs(V) > lm / rho) ^ (rng.random(V.shape) < 0.02)
+               for _, V in st['tape']]
+    Yn, _ = ur.forward_numpy(D, S, lm, rho, n, supports=sup)
+    supt = None if sup is None else [torch.tensor(s) for s in sup]
+    Yt, _ = ur.forward_tape_torch(torch.tensor(D), torch.tensor(S), lm, rho,
+                                  n, supports=supt)
+    assert np.allclose(Yt.numpy(), Yn, rtol=0, atol=1e-12)
+    d, s, l = ur.adjoint_numpy(D, S, lm, rho, n, G, supports=sup)
+    dt, st_, lt = ur.adjoint_torch(torch.tensor(D), torch.tensor(S), lm, rho,
+                                   n, torch.tensor(G), supports=supt)
+    assert np.allclose(dt.numpy(), d, rtol=0, atol=1e-11)
+    assert np.allclose(st_.numpy(), s, rtol=0, atol=1e-11)
+    assert np.isclose(lt, l, rtol=1e-11)
