@@ -4,10 +4,13 @@
 One step is one ADMM iteration of the plain CBPDN solver (forward column
 FFTs, Sherman-Morrison X-solve, inverse column FFTs, fused row kernel,
 residual reduction) on a synthetic image and dictionary resident in HBM,
-called through the C ABI's device-pointer entry point.  The timed region is
-one solve of exactly --steps iterations (fixed rho, AutoRho off, zero
-stopping tolerances), so it also contains the solve's setup (spectra of D
-and S), which makes the figure slightly conservative.
+called through the C ABI's device-pointer entry point (fixed rho, AutoRho
+off, zero stopping tolerances, so exactly --steps iterations run; checked).
+The per-iteration time is taken the way the reference's own timing script
+does (examples/time_cbpdn.py:43-59): a solve of --steps iterations minus a
+solve of 0 iterations (setup only: spectra of D and S, the final
+Y = shrink(V)), each the median of --repeats timed calls, every call
+bracketed by device synchronisations.
 
 The JSON line carries:
   roofline      the dominant kernel's algorithmic bytes per launch over its
@@ -15,12 +18,20 @@ The JSON line carries:
                 the 8 TB/s HBM peak; traffic from the committed rocprofv3
                 counter summary when one exists for the workload;
   cpu_baseline  the FP32 numpy/scipy restatement of the same loop (the
-                oracle) timed on the host for a bounded number of iterations;
+                oracle) timed on the host for a bounded number of
+                iterations, and the same in FP64;
   iteration_model  the whole iteration against the 4-stage compulsory
                 traffic model 32 M N + 40 M Hc + 12 Hc bytes (the solver
                 moves 8 M N + 72 M Hc + 8 Hc: its state is V = Xr + U, one
-                real array, instead of Y and U).
-  kernels       per-kernel average duration and algorithmic GB/s.
+                real array, instead of Y and U);
+  kernels       per-kernel average duration and algorithmic GB/s;
+  configs       the same steady-state rate for the other benchmark
+                configurations (256 x 256 x 32 plain; 512 x 512 x 64 with
+                the gradient term, with and without mask; 2048 x 2048 x
+                128 plain and with the gradient term);
+  accuracy      the per-configuration errors against the FP64 loop that
+                tests/test_gpu_configs.py measured on this tree
+                (profiles/parity_configs.json), when present.
 """
 
 import argparse
@@ -44,6 +55,26 @@ WORKLOADS = {
                              relax=1.8, seed=0),
     'plain-2048x2048x128': dict(H=2048, W=2048, K=16, M=128, lmbda=0.02,
                                 rho=100.0, relax=1.8, seed=2),
+}
+
+# The other benchmark configurations, timed the same way (fixed iterations
+# of each configuration; rho = 50 lambda + 1 where the configuration names
+# none, as the drop-in API defaults).
+CONFIGS = {
+    'cfg0 plain 256x256x32': dict(H=256, W=256, K=8, M=32, lmbda=0.05,
+                                  rho=50.0, relax=1.8, seed=0, steps=100),
+    'cfg1 grad 512x512x64': dict(H=512, W=512, K=12, M=64, lmbda=0.01,
+                                 rho=None, relax=1.8, seed=1, steps=200,
+                                 mu=0.5),
+    'cfg1 grad+mask 512x512x64': dict(H=512, W=512, K=12, M=64, lmbda=0.01,
+                                      rho=None, relax=1.8, seed=1, steps=200,
+                                      mu=0.5, mask=True),
+    'cfg2 plain 2048x2048x128': dict(H=2048, W=2048, K=16, M=128,
+                                     lmbda=0.02, rho=100.0, relax=1.8,
+                                     seed=2, steps=100),
+    'cfg2 grad 2048x2048x128': dict(H=2048, W=2048, K=16, M=128, lmbda=0.02,
+                                    rho=100.0, relax=1.8, seed=2, steps=100,
+                                    mu=0.3),
 }
 
 
@@ -78,11 +109,26 @@ def synth_dict(K, M, seed):
     return np.ascontiguousarray(np.rollaxis(D.astype(np.float32), 2, 0))
 
 
-def model_bytes_iter(M, R, C):
-    """Compulsory fp32 bytes of one iteration of the 4-stage form."""
+def synth_mask(H, W, seed, p_observed=0.7):
+    """Bernoulli(0.7) observed pixels (1 = observed), int32."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    return (rng.random((H, W)) < p_observed).astype(np.int32)
+
+
+def model_bytes_iter(M, R, C, grad=False, mask=False):
+    """Compulsory fp32 bytes of one iteration of the 4-stage form (M maps
+    excluding the impulse map of a masked solve)."""
     N = R * C
     Hc = R * (C // 2 + 1)
-    return 32 * M * N + 40 * M * Hc + 12 * Hc
+    if mask:
+        M += 1
+    b = 32 * M * N + 40 * M * Hc + 12 * Hc
+    if grad:
+        b += 4 * Hc
+    if mask:
+        b += 4 * N
+    return b
 
 
 def stage_bytes(M, R, C):
@@ -97,8 +143,9 @@ def stage_bytes(M, R, C):
     }
 
 
-def cpu_baseline(wl, D, S, budget_s):
-    """FP32 oracle loop on the host cores for about budget_s seconds."""
+def cpu_baseline(wl, D, S, budget_s, dtype='float32'):
+    """Oracle loop (FP32 or FP64) on the host cores for about budget_s
+    seconds."""
     import numpy as np
     from oracle import cbpdn as ocb
     cores = len(os.sched_getaffinity(0))
@@ -113,8 +160,8 @@ def cpu_baseline(wl, D, S, budget_s):
             raise _Stop()
     try:
         ocb.solve(D, S, wl['lmbda'], rho=wl['rho'], max_iter=10000,
-                  relax_param=wl['relax'], rel_tol=0.0, dtype=np.float32,
-                  workers=cores, callback=cb)
+                  relax_param=wl['relax'], rel_tol=0.0,
+                  dtype=np.dtype(dtype), workers=cores, callback=cb)
     except _Stop:
         pass
     n = len(stamps) - 1
@@ -123,9 +170,123 @@ def cpu_baseline(wl, D, S, budget_s):
         'value': 1.0 / per_it, 'unit': 'ADMM iterations/s', 'cores': cores,
         'kind': 'port',
         'sample': ('%d iterations of the same workload (after the first), '
-                   'FP32 numpy + scipy.fft with workers=%d; numpy '
-                   'elementwise work single-threaded') % (n, cores),
+                   '%s numpy + scipy.fft with workers=%d; numpy '
+                   'elementwise work single-threaded')
+                  % (n, dtype.upper().replace('FLOAT', 'FP'), cores),
     }
+
+
+class DeviceProblem:
+    """One configuration's inputs in HBM and its solve through
+    scc_cbpdn_device (fixed iterations, fixed rho, AutoRho off)."""
+
+    def __init__(self, c):
+        import numpy as np
+        import torch
+        from sporco_cuda import _abi
+        self.c = c
+        R, C, M, K = c['H'], c['W'], c['M'], c['K']
+        S = synth_image(R, C, 1000 + c['seed'])
+        D = synth_dict(K, M, 2000 + c['seed'])
+        self.grad = 'mu' in c
+        self.mask = bool(c.get('mask'))
+        lf = float(np.float32(c['lmbda']))
+        self.rho = 50.0 * lf + 1.0 if c['rho'] is None else c['rho']
+        dev = torch.device('cuda:0')
+        Mt = M
+        l1 = np.ones(1, np.float32)
+        gw = np.ones(1, np.float32)
+        W = None
+        if self.mask:
+            # cbpdnmsk / cbpdngrdmsk: impulse filter first, with zero L1
+            # and gradient weight, image W * s (cbpdn.pyx:113-137, 259-275).
+            W = synth_mask(R, C, 3000 + c['seed'])
+            S = (W * S).astype(np.float32)
+            imp = np.zeros((1, K, K), np.float32)
+            imp[0, 0, 0] = 1.0
+            D = np.concatenate([imp, D])
+            Mt = M + 1
+            l1 = np.ones(Mt, np.float32)
+            l1[0] = 0.0
+            gw = np.ones(Mt, np.float32)
+            gw[0] = 0.0
+        self.Mt = Mt
+        self.D = torch.from_numpy(np.ascontiguousarray(D)).to(dev)
+        self.S = torch.from_numpy(S).to(dev)
+        self.l1 = torch.from_numpy(l1).to(dev)
+        self.gw = torch.from_numpy(gw).to(dev)
+        self.W = None if W is None else torch.from_numpy(W).to(dev)
+        self.Y = torch.empty((Mt, R, C), dtype=torch.float32, device=dev)
+        self.L = _abi.lib()
+        self.abi = _abi
+        self.stream = torch.cuda.current_stream().cuda_stream
+
+    def solve(self, iters):
+        import ctypes
+        c, _abi = self.c, self.abi
+        R, C, K = c['H'], c['W'], c['K']
+        o = _abi.make_opt(R, C, self.Mt, K, K, self.rho, iters,
+                          relax=c['relax'], l1_ptr=self.l1.data_ptr(),
+                          l1_size=(self.l1.numel(), 1, 1),
+                          gw_ptr=self.gw.data_ptr(),
+                          gw_size=self.gw.numel(),
+                          mask_ptr=None if self.W is None
+                          else self.W.data_ptr())
+        rc = self.L.scc_cbpdn_device(
+            self.D.data_ptr(), self.S.data_ptr(), c['lmbda'],
+            c.get('mu', 0.0), 1 if self.grad else 0, ctypes.byref(o),
+            self.Y.data_ptr(), self.stream)
+        if rc != 0:
+            raise RuntimeError('solve failed (%d): %s'
+                               % (rc, _abi.last_error()))
+        return _abi.last_info()
+
+    def timed(self, iters):
+        import torch
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        info = self.solve(iters)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        if info['iterations'] != iters:
+            raise RuntimeError('solve ran %d of %d iterations'
+                               % (info['iterations'], iters))
+        return t
+
+    def steady_rate(self, steps, repeats):
+        """Iterations per second of the solve loop: (median time of a
+        `steps`-iteration solve - median time of a 0-iteration solve)."""
+        import statistics
+        tk, t0 = [], []
+        for _ in range(repeats):
+            t0.append(self.timed(0))
+            tk.append(self.timed(steps))
+        mk, m0 = statistics.median(tk), statistics.median(t0)
+        return {'value': steps / (mk - m0), 'ms_per_iter':
+                1000.0 * (mk - m0) / steps, 'call_ms_median': 1000.0 * mk,
+                'setup_call_ms_median': 1000.0 * m0,
+                'call_ms_all': [1000.0 * t for t in tk],
+                'one_call_iters_per_s': steps / mk}
+
+
+def configs_leg(repeats):
+    """Steady-state rate of every other benchmark configuration."""
+    out = {}
+    for name, c in CONFIGS.items():
+        print('bench: %s, %d iterations' % (name, c['steps']),
+              file=sys.stderr, flush=True)
+        pb = DeviceProblem(c)
+        pb.solve(c['steps'])
+        pb.solve(c['steps'])
+        r = pb.steady_rate(c['steps'], repeats)
+        mb = model_bytes_iter(c['M'], c['H'], c['W'], pb.grad, pb.mask)
+        r.update(steps=c['steps'], bytes_per_iter_model=mb,
+                 model_GBps=mb * r['value'] / 1e9,
+                 model_frac=mb * r['value'] / 1e9 / HBM_PEAK_GBPS)
+        del r['call_ms_all']
+        out[name] = r
+        del pb
+    return out
 
 
 def batched_leg(seed=3, n_images=64, iters=150):
@@ -311,6 +472,10 @@ def main():
     ap.add_argument('--no-plain', action='store_true',
                     help='skip the plain-solver timing (secondary legs only)')
     ap.add_argument('--batched-iters', type=int, default=150)
+    ap.add_argument('--repeats', type=int, default=5,
+                    help='timed calls per length (median)')
+    ap.add_argument('--no-configs', action='store_true',
+                    help='skip the other configurations')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='only run 4 device copies of 256 MiB (the reference '
                          'kernel that scales the FETCH_SIZE / WRITE_SIZE '
@@ -344,42 +509,23 @@ def main():
 
     wl = WORKLOADS[args.workload]
     R, C, M, K = wl['H'], wl['W'], wl['M'], wl['K']
-    S_h = synth_image(R, C, 1000 + wl['seed'])
-    D_h = synth_dict(K, M, 2000 + wl['seed'])
-    dev = torch.device('cuda:0')
-    D = torch.from_numpy(D_h).to(dev)
-    S = torch.from_numpy(S_h).to(dev)
-    w = torch.ones(1, dtype=torch.float32, device=dev)
-    Y = torch.empty((M, R, C), dtype=torch.float32, device=dev)
-    L = _abi.lib()
-    stream = torch.cuda.current_stream().cuda_stream
-
-    def solve(iters):
-        o = _abi.make_opt(R, C, M, K, K, wl['rho'], iters, relax=wl['relax'],
-                          l1_ptr=w.data_ptr())
-        rc = L.scc_cbpdn_device(D.data_ptr(), S.data_ptr(), wl['lmbda'], 0.0,
-                                0, ctypes.byref(o), Y.data_ptr(), stream)
-        if rc != 0:
-            raise RuntimeError('solve failed (%d): %s' % (rc,
-                                                          _abi.last_error()))
-
+    pb = DeviceProblem(wl)
     print('bench: %s warmup %d iterations' % (args.workload, args.warmup),
           file=sys.stderr, flush=True)
-    solve(args.warmup)
+    pb.solve(args.warmup)
+    pb.solve(args.steps)
     torch.cuda.synchronize()
-    print('bench: timing %d iterations' % args.steps, file=sys.stderr,
-          flush=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    solve(args.steps)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
+    print('bench: timing %d iterations, %d repeats' % (args.steps,
+                                                       args.repeats),
+          file=sys.stderr, flush=True)
+    rate = pb.steady_rate(args.steps, args.repeats)
+    value = rate['value']
     info = _abi.last_info()
-    value = args.steps / (t1 - t0)
 
-    # Per-kernel durations over a second run of the same length.
+    # Per-kernel durations over a further run of the same length.
+    L = _abi.lib()
     L.scc_set_stage_timing(1)
-    solve(args.steps)
+    pb.solve(args.steps)
     torch.cuda.synchronize()
     L.scc_set_stage_timing(0)
     st = _abi.last_stage_times()
@@ -388,7 +534,7 @@ def main():
     for k, (ms, n) in st.items():
         if n:
             avg = ms / n
-            per_kernel[k] = {'avg_us': 1000 * avg}
+            per_kernel[k] = {'avg_us': 1000 * avg, 'launches': n}
             if k in sb:
                 per_kernel[k]['GBps'] = sb[k] / (avg * 1e-3) / 1e9
     dom = max((k for k in per_kernel if k in sb), key=lambda k: st[k][0])
@@ -409,7 +555,10 @@ def main():
                                '%dx%d, lambda=%g, rho=%g, RelaxParam=%g, '
                                'AutoRho off, fixed iterations'
                                % (args.workload, R, C, M, K, K, wl['lmbda'],
-                                  wl['rho'], wl['relax'])},
+                                  wl['rho'], wl['relax']),
+                   'timing': '(median %d-iteration solve - median '
+                             '0-iteration solve) over %d repeats'
+                             % (args.steps, args.repeats)},
         'roofline': roofline,
         'iteration_model': {'bytes_per_iter': mb,
                             'achieved_GBps': mb * value / 1e9,
@@ -417,15 +566,34 @@ def main():
                             'device_solve_ms_per_iter':
                                 info['solve_ms'] / max(info['iterations'], 1)},
         'kernels': per_kernel,
+        'timing': rate,
         'setup_ms': info['setup_ms'],
     }
+    del pb
+    out['configs'] = None if args.no_configs else configs_leg(args.repeats)
     out['batched'] = None if args.no_batched else batched_leg(
         iters=args.batched_iters)
     out['adjoint'] = None if args.no_adjoint else adjoint_leg()
+    acc = os.path.join(ROOT, 'profiles', 'parity_configs.json')
+    if os.path.exists(acc):
+        with open(acc) as f:
+            out['accuracy'] = json.load(f)
     if not args.no_cpu:
-        print('bench: cpu baseline (%.0f s budget)' % args.cpu_seconds,
-              file=sys.stderr, flush=True)
-        out['cpu_baseline'] = cpu_baseline(wl, D_h, S_h, args.cpu_seconds)
+        print('bench: cpu baseline (%.0f s budget, FP32 then FP64)'
+              % args.cpu_seconds, file=sys.stderr, flush=True)
+        S_h = synth_image(R, C, 1000 + wl['seed'])
+        D_h = synth_dict(K, M, 2000 + wl['seed'])
+        cpu = cpu_baseline(wl, D_h, S_h, args.cpu_seconds)
+        cpu['fp64'] = cpu_baseline(wl, D_h, S_h, args.cpu_seconds,
+                                   dtype='float64')
+        c0 = CONFIGS['cfg0 plain 256x256x32']
+        S0 = synth_image(c0['H'], c0['W'], 1000 + c0['seed'])
+        D0 = synth_dict(c0['K'], c0['M'], 2000 + c0['seed'])
+        cpu['cfg0 plain 256x256x32'] = {
+            'fp32': cpu_baseline(c0, D0, S0, args.cpu_seconds / 3),
+            'fp64': cpu_baseline(c0, D0, S0, args.cpu_seconds / 3,
+                                 dtype='float64')}
+        out['cpu_baseline'] = cpu
     else:
         out['cpu_baseline'] = None
     print(json.dumps(out), flush=True)

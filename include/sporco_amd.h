@@ -146,6 +146,19 @@ int scc_last_objective(float *rows, int cap);
 void scc_set_stage_timing(int enabled);
 int scc_last_stage_times(double *ms_total, int *launches, int n);
 
+/* Graph replay of repeated fixed-iteration solves on this thread (default
+ * on): a solve with no stopping tolerance, no Verbose output, objective or
+ * stage timing whose configuration (sizes, parameters, buffers, weight and
+ * mask pointers, iteration count) equals the previous such solve in the
+ * same cached workspace runs its iterations from a captured HIP graph
+ * (kernel nodes only), captured on the second call and replayed on later
+ * ones; the results are those of direct launches.  scc_last_solve_graph
+ * returns 1 when the last solve on this thread ran from a graph.  No
+ * reference counterpart (the reference launches every kernel from the
+ * host and synchronises twice per iteration, cbpdn.cu:318-567). */
+void scc_set_solve_graph(int enabled);
+int scc_last_solve_graph(void);
+
 /* Differentiable unrolled plain CBPDN over a batch of B images of one
  * power-of-two size (sides 256..2048) sharing a dictionary: n_iter ADMM
  * iterations with fixed rho, relaxation alpha, scalar lambda, L1 weight 1.

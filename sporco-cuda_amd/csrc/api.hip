@@ -96,6 +96,10 @@ thread_local std::vector<float> g_obj;  // (Fnc, DFid, Regl1, Regl2g) rows
 enum Stage { kStColFwd = 0, kStSolve, kStColInv, kStRow, kStReduce, kStRowFwd,
              kNumStages };
 thread_local bool g_stage_on = false;
+// Graph replay of repeated fixed-iteration solves (scc_set_solve_graph), and
+// whether the last solve on this thread ran from a graph.
+thread_local bool g_graph_on = true;
+thread_local int g_graph_used = 0;
 thread_local double g_stage_ms[kNumStages];
 thread_local int g_stage_n[kNumStages];
 
@@ -128,7 +132,9 @@ struct DevBuf {
 int ensure(DevBuf& b, size_t bytes) {
   if (b.bytes >= bytes && (b.p || bytes == 0)) return SCC_OK;
   if (!b.owned)
-    return fail(SCC_ERR_UNSUPPORTED, "internal: device buffer view too small");
+    return fail(SCC_ERR_UNSUPPORTED,
+                "buffer needed beyond the fixed workspace of an "
+                "allocation-free solve");
   if (b.p) {
     (void)hipFree(b.p);
     b.p = nullptr;
@@ -144,6 +150,15 @@ int ensure(DevBuf& b, size_t bytes) {
   b.bytes = bytes;
   return SCC_OK;
 }
+
+// What a captured iteration loop depends on: the kernels' parameters and
+// buffers (caller weight and mask pointers included) and the iteration
+// count.  Value-initialised, so that padding bytes compare equal.
+struct GraphKey {
+  Params p;
+  Buffers b;
+  int maxit;
+};
 
 // Everything that depends only on (device, R, C, M): buffers, plans, tables.
 // The entry points cache one per device; one whose buffers are views into
@@ -165,8 +180,15 @@ struct Workspace {
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   Ctrl* hctrl = nullptr;  // host copy of the control block (pinned if owned)
   std::mutex mu;          // held for the whole of a solve in this workspace
+  // Captured iteration loop of the last repeated fixed-iteration solve, and
+  // the configuration of the last solve that was not yet captured.
+  hipGraphExec_t gexec = nullptr;
+  GraphKey gkey{};
+  GraphKey seen{};
+  bool seen_valid = false;
   ~Workspace() {
     if (!owned) return;
+    if (gexec) (void)hipGraphExecDestroy(gexec);
     if (hctrl) (void)hipHostFree(hctrl);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
@@ -436,6 +458,7 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
   const scc_algopt& o = rs.o;
   const Geom& g = ws->g;
   const int M = g.M;
+  g_graph_used = 0;
   Params p{};
   p.lambda = lambda;
   p.mu = mu;
@@ -553,9 +576,9 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
     SCC_HIP(launch_col(g, 1, ws->colp, b.Sf, b.twR, 0, nullptr, st));
   }
   // Y = U = 0 and A-hat = 0 for the first iteration (cbpdn.cu:321-326).
-  SCC_HIP(hipMemsetAsync(b.Y, 0, nreal * 4, st));
-  if (need_u) SCC_HIP(hipMemsetAsync(b.U, 0, nreal * 4, st));
-  SCC_HIP(hipMemsetAsync(b.F, 0, ncplx * 8, st));
+  SCC_HIP(launch_zero(b.Y, nreal * 4, st));
+  if (need_u) SCC_HIP(launch_zero(b.U, nreal * 4, st));
+  SCC_HIP(launch_zero(b.F, ncplx * 8, st));
   Ctrl c0{};
   c0.rho = o.rho;
   c0.rsf = 1.f;
@@ -600,6 +623,7 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
   // the solve, before the inverse transforms and the row kernel test the
   // stopping flag), saving two launches.
   bool pending = false;
+  auto loop = [&]() -> int {
   for (int i = 1; i <= maxit; ++i) {
     const bool rho_step = p.autorho && i > 1 && i % p.period == 0;
     const int rmode = rho_step ? kRowUpdate : kRowFused;
@@ -681,6 +705,56 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
       }
       if (ws->hctrl->done) break;
     }
+  }
+  return SCC_OK;
+  };
+  // Fixed-iteration solves (no host poll, objective or stage timing) run
+  // their iterations from a captured HIP graph once the same configuration
+  // (parameters, buffers, iteration count) has been solved before: the
+  // second such call captures and instantiates it, later calls only replay
+  // it, which removes the per-launch host cost (4 launches per iteration).
+  // The graph holds kernel nodes only (no memset or copy nodes).
+  const bool graphable = g_graph_on && ws->owned && !stage_on && !obj_on &&
+                         !can_stop && o.Verbose != 1 && maxit > 0;
+  if (graphable) {
+    GraphKey key;
+    std::memset(&key, 0, sizeof(key));
+    std::memcpy(&key.p, &p, sizeof(p));
+    std::memcpy(&key.b, &b, sizeof(b));
+    key.maxit = maxit;
+    if (ws->gexec != nullptr && std::memcmp(&key, &ws->gkey, sizeof(key)) ==
+                                    0) {
+      SCC_HIP(hipGraphLaunch(ws->gexec, st));
+      g_graph_used = 1;
+    } else if (ws->seen_valid &&
+               std::memcmp(&key, &ws->seen, sizeof(key)) == 0) {
+      SCC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      rc = loop();
+      hipGraph_t graph = nullptr;
+      hipError_t ce = hipStreamEndCapture(st, &graph);
+      if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      if (ce != hipSuccess)
+        return fail(SCC_ERR_HIP, "hipStreamEndCapture", hipGetErrorString(ce));
+      hipGraphExec_t exec = nullptr;
+      hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ie != hipSuccess)
+        return fail(SCC_ERR_HIP, "hipGraphInstantiate", hipGetErrorString(ie));
+      if (ws->gexec) (void)hipGraphExecDestroy(ws->gexec);
+      ws->gexec = exec;
+      ws->gkey = key;
+      SCC_HIP(hipGraphLaunch(ws->gexec, st));
+      g_graph_used = 1;
+    } else {
+      ws->seen = key;
+      ws->seen_valid = true;
+      if ((rc = loop())) return rc;
+    }
+  } else {
+    if ((rc = loop())) return rc;
   }
   if (ws->ev[2]) SCC_HIP(hipEventRecord(ws->ev[2], st));
   if (p.vstate)  // Y = shrink(V), into the output or in place
@@ -893,6 +967,10 @@ int scc_last_history(float* rsr, int cap) {
 }
 
 void scc_set_stage_timing(int enabled) { g_stage_on = enabled != 0; }
+
+void scc_set_solve_graph(int enabled) { g_graph_on = enabled != 0; }
+
+int scc_last_solve_graph(void) { return g_graph_used; }
 
 int scc_last_stage_times(double* ms_total, int* launches, int n) {
   for (int k = 0; k < n && k < kNumStages; ++k) {

@@ -390,6 +390,31 @@ hipError_t launch_ghg(const Geom& g, float* ghg, hipStream_t st) {
   return hipGetLastError();
 }
 
+// Zero fill of n 32-bit words (a kernel rather than a memset node, so that
+// the clear can sit in a captured graph).  Four words per store where the
+// pointer is 16-byte aligned.
+__global__ __launch_bounds__(256) void k_zero(unsigned* p, long long n) {
+  const long long stride = (long long)gridDim.x * 256;
+  long long i = blockIdx.x * 256LL + threadIdx.x;
+  if (((uintptr_t)p & 15) == 0) {
+    uint4* q = reinterpret_cast<uint4*>(p);
+    const long long n4 = n >> 2;
+    for (long long k = i; k < n4; k += stride) q[k] = make_uint4(0, 0, 0, 0);
+    for (long long k = (n4 << 2) + i; k < n; k += stride) p[k] = 0u;
+  } else {
+    for (long long k = i; k < n; k += stride) p[k] = 0u;
+  }
+}
+
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t st) {
+  const long long n = (long long)(bytes / 4);
+  if (n == 0) return hipSuccess;
+  const unsigned nb =
+      (unsigned)std::min<long long>((n / 4 + 255) / 256 + 1, 8192);
+  hipLaunchKernelGGL(k_zero, dim3(nb), dim3(256), 0, st, (unsigned*)p, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_any_negative(const float* w, long long n, int* flag,
                                hipStream_t st) {
   hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, st, flag);

@@ -153,6 +153,8 @@ hipError_t launch_any_negative(const float* w, long long n, int* flag,
 // flag[0] = 1 if any of the n mask entries is neither 0 nor 1, else 0.
 hipError_t launch_any_nonbinary(const int* mask, long long n, int* flag,
                                 hipStream_t st);
+// Zero fill of bytes / 4 32-bit words at p (a kernel, not a memset).
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t st);
 // Twiddle table tw[t] = exp(-2 pi i t / L), t < L, from double-precision
 // cos / sin rounded to float.
 hipError_t launch_twiddles(float2* tw, int L, hipStream_t st);

@@ -100,6 +100,8 @@ def lib():
         'scc_get_device_name': (ctypes.c_char_p, [_i]),
         'scc_release_cache': (None, []),
         'scc_set_stage_timing': (None, [_i]),
+        'scc_set_solve_graph': (None, [_i]),
+        'scc_last_solve_graph': (_i, []),
         'scc_unrolled_ckpt_floats': (ctypes.c_size_t, [_i, _i, _i, _i, _i,
                                                        _i]),
         'scc_unrolled_forward': (_i, [vp, _i, _i, vp, _i, _i, _i, _f, _f, _f,

@@ -5,12 +5,14 @@ sporco_cuda/tests/test_cbpdn.py plus option cases) is solved on the GPU
 through sporco_cuda.cbpdn -> C ABI -> HIP kernels and compared with the
 stored FP64 oracle result.
 
-Tolerances: relative L2 error <= max(1e-4, 2 e32), where e32 is the error
+Tolerances: relative L2 error <= max(2e-5, 10 e32), where e32 is the error
 of the same oracle run in FP32 against FP64 (the FP32 floor of the case;
 cases with AutoRho at every iteration amplify rounding to ~1e-4), and the
 reference's own bar, mean squared error < 1e-8 (1e-6 for the full-array
 L1 weight case, test_cbpdn.py:90, whose N(0, 1) weights include negative
-ones and make the problem non-convex: relative error <= 1e-3 there).  Where a stopping rule ends the solve,
+ones: there the soft threshold jumps at V = 0 and the result is compared
+on the branch of those jumps the GPU took, see fp64_branch).  Where a
+stopping rule ends the solve,
 the GPU's fp32 residuals may cross the threshold one iteration earlier or
 later than the FP64 ones; the GPU result is then compared with the oracle
 run for the GPU's iteration count.
@@ -76,6 +78,61 @@ def rel_err(a, b):
         np.linalg.norm(b), 1e-30)
 
 
+def fp64_branch(c, gpu_hist, tol=2e-5, band=1e-5, max_flips=6):
+    """The FP64 oracle on the branch of the soft threshold the GPU took.
+
+    With a negative L1 weight w the soft threshold of a coefficient,
+    copysign(max(|V| - t, 0), V) with t = lambda w / rho < 0, jumps by
+    2 |t| where V crosses 0, so where V lies within FP32 rounding of 0 the
+    GPU's FP32 iterate and the FP64 oracle can take different sides, and
+    the iterates then move apart by O(|t|).  Starting from the plain FP64
+    run: find the first iteration whose residuals (r, s) differ from the
+    GPU's recorded history by more than tol (relative; FP32 rounding moves
+    them by ~2e-6 on this case, one inverted decision by ~1.5e-4), take the
+    negative-weight coefficient with the smallest |V| of that iteration,
+    require |V| <= band max|V| (a sign decision that rounding can change),
+    invert its sign decision there (oracle.cbpdn.solve's ``invert``), and
+    repeat until the histories agree.  Returns (Y, inverted decisions as
+    (iteration, index, V / max|V|) triples); fails when a divergence is not
+    explained by such a sign decision."""
+    from oracle import cbpdn as ocb
+    assert c['api'] in ('cbpdn', 'cbpdngrd')
+    Dm = np.ascontiguousarray(np.rollaxis(c['D'], 2, 0))
+    kw = ocb._core_kwargs(Dm, c['S'], c['lmbda'], c['opt'],
+                          c['api'] == 'cbpdngrd',
+                          c['mu'] if c['api'] == 'cbpdngrd' else None)
+    M, R, C = Dm.shape[0], c['S'].shape[0], c['S'].shape[1]
+    w = np.broadcast_to(np.asarray(kw['l1_weight'], np.float64), (M, R, C))
+    invert, flips = {}, []
+    g = np.asarray(gpu_hist, np.float64)[:, :2]
+    for _ in range(max_flips + 1):
+        Vs = {}
+
+        def cb(i, X, Y, U):
+            Vs[i] = Y + U
+        Y, info = ocb.solve(Dm, c['S'], c['lmbda'], track=True,
+                            invert=invert, callback=cb, **kw)
+        h = np.asarray(info['history'], np.float64)[:, :2]
+        n = min(len(h), len(g))
+        d = np.abs(h[:n] - g[:n]) / np.maximum(np.abs(h[:n]), 1e-30)
+        bad = np.nonzero(np.any(d > tol, axis=1))[0]
+        if len(bad) == 0:
+            return np.rollaxis(Y, 0, 3), flips
+        k = int(bad[0]) + 1
+        V = Vs[k]
+        m = np.where(w < 0, np.abs(V), np.inf)
+        j = int(np.argmin(m))
+        frac = float(m.flat[j] / np.max(np.abs(V)))
+        assert frac <= band, ('divergence at iteration %d (relative '
+                              'deviations %s) not explained by a sign '
+                              'decision near V = 0 (%.2e)'
+                              % (k, np.array2string(d[max(k - 3, 0):k + 2]),
+                                 frac))
+        invert.setdefault(k, []).append(j)
+        flips.append((k, j, float(V.flat[j]) / float(np.max(np.abs(V)))))
+    raise AssertionError('more than %d inverted sign decisions' % max_flips)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize('path', FIXTURES,
                          ids=[os.path.basename(p)[:-4] for p in FIXTURES])
@@ -83,7 +140,12 @@ def test_golden(gpu, path):
     import sporco_cuda.cbpdn as cb
     from oracle import cbpdn as ocb
     c = load_case(path)
-    X = run_api(cb, c)
+    cb.set_history(True)
+    try:
+        X = run_api(cb, c)
+        hist = cb.last_history()
+    finally:
+        cb.set_history(False)
     info = cb.last_info()
     assert X.shape == c['X'].shape
     assert X.dtype == np.float32
@@ -98,18 +160,24 @@ def test_golden(gpu, path):
     err = rel_err(X, expected)
     mse = np.mean((X - expected) ** 2)
     e32 = rel_err(run_api(_Oracle32(), c, opt), expected)
-    # ref04 draws an L1 weight for every coefficient from N(0, 1), as the
-    # reference's own test does: negative weights make the problem
-    # non-convex, and after ~20 iterations FP32 iterates that agree to
-    # 3e-7 can take locally different paths (two builds differing only in
-    # FFT rounding end 4.7e-4 apart).  Its bounds are 1e-3 and MSE 1e-6.
+    name = os.path.basename(path)[:-4]
     print('%s: rel. L2 error %.3e, FP32 oracle %.3e, MSE %.3e'
-          % (os.path.basename(path)[:-4], err, e32, mse))
-    sensitive = os.path.basename(path).startswith('ref04')
-    tol_mse = 1e-6 if sensitive else 1e-8
-    tol_rel = 1e-3 if sensitive else 1e-4
-    assert err <= max(tol_rel, 2 * e32) and mse < tol_mse, (err, e32, mse,
-                                                             info)
+          % (name, err, e32, mse))
+    bar = max(2e-5, 10 * e32)
+    if name.startswith('ref04'):
+        # The reference's own test 04 (an N(0, 1) L1 weight per
+        # coefficient, negative ones included) holds sporco's result to MSE
+        # 1e-6 (test_cbpdn.py:90); the soft threshold of a negative weight
+        # jumps at V = 0, and the GPU's result is checked on the branch of
+        # those jumps it took (fp64_branch).
+        assert mse < 1e-6, (err, mse)
+        Xb, flips = fp64_branch(c, hist)
+        eb = rel_err(X, Xb)
+        print('%s: %d inverted sign decision(s) %s; rel. L2 error on the '
+              "GPU's branch %.3e" % (name, len(flips), flips, eb))
+        assert eb <= bar, (eb, e32, flips)
+        return
+    assert err <= bar and mse < 1e-8, (err, e32, mse, info)
 
 
 @pytest.mark.gpu
@@ -180,4 +248,39 @@ def test_nonbinary_mask_autorho(gpu):
     err, e32 = rel_err(X, Xo), rel_err(X32, Xo)
     print('non-binary mask + AutoRho: rel. error %.3e (FP32 oracle %.3e)'
           % (err, e32))
-    assert err <= max(1e-4, 10 * e32), (err, e32)
+    assert err <= max(2e-5, 10 * e32), (err, e32)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('shape', [(32, 31), (256, 256)],
+                         ids=['generic', 'pow2'])
+@pytest.mark.parametrize('k', [0, 1, 2])
+def test_autorho_sequence_scalar_problem(gpu, shape, k):
+    """The rho sequence of AutoRho on the closed-form scalar problem of
+    tests/test_oracle.py (impulse dictionary, constant image: every
+    coefficient runs the same scalar recursion, and the penalty follows the
+    residual-balancing rule of cbpdn.cu:521-559), on the GPU."""
+    import sporco_cuda.cbpdn as cb
+    from test_oracle import AUTORHO_SCALAR, scalar_autorho
+    sv, lm, rho, alpha, n, period, tau, mu, xi, asc = AUTORHO_SCALAR[k]
+    expect = scalar_autorho(sv, lm, rho, alpha, n, period, tau, mu, xi, asc)
+    D = np.ones((1, 1, 1), np.float32)
+    S = np.full(shape, sv, np.float32)
+    opt = cb.default_options({
+        'MaxMainIter': n, 'rho': rho, 'RelaxParam': alpha,
+        'RelStopTol': 0.0, 'AbsStopTol': 0.0,
+        'AutoRho': {'Enabled': True, 'Period': period, 'AutoScaling': asc,
+                    'Scaling': tau, 'RsdlRatio': mu, 'RsdlTarget': xi}})
+    cb.set_history(True)
+    try:
+        cb.cbpdn(D, S, lm, opt)
+        h = cb.last_history()
+    finally:
+        cb.set_history(False)
+    got = h[:, 2].astype(np.float64)
+    print('AutoRho scalar case %d %s: rho %s' % (k, shape, np.round(got, 5)))
+    assert len(got) == n
+    # FP32 residual sums over the image's identical pixels set the
+    # autoscaled factors to ~1e-5; a swapped branch or a wrong factor
+    # moves rho by a factor of 2 or more.
+    assert np.allclose(got, expect, rtol=1e-4, atol=0), (got, expect)

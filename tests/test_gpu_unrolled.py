@@ -7,7 +7,7 @@ coefficients a few values of V lie within FP32 rounding (~1e-6) of t, and
 there the FP32 GPU forward and the FP64 oracle can take different branches;
 one flipped support moves the gradient by about 1 / sqrt(#coefficients)
 (~1e-3 relative), so gradients are compared on the branch the GPU took:
-  * forward Y against the FP64 oracle (relative L2 <= max(1e-4, 2 e32));
+  * forward Y against the FP64 oracle (relative L2 <= max(2e-5, 10 e32));
   * the GPU's V of every iteration (scc_unrolled_tape) against FP64, and
     every support that differs lies within 1e-5 of t (a rounding flip);
   * gradients w.r.t. D, S, lambda against the FP64 hand adjoint on the
@@ -70,7 +70,9 @@ def test_unrolled_forward_and_gradients(gpu):
     Y32, _ = ur.forward_numpy(D.astype(np.float32), S.astype(np.float32),
                               np.float32(LAM), np.float32(RHO), NIT,
                               np.float32(ALPHA))
-    assert rel(Y, Yo) <= max(1e-4, 2 * rel(Y32, Yo))
+    e32 = rel(Y32, Yo)
+    print('forward: rel. L2 error %.3e, FP32 oracle %.3e' % (rel(Y, Yo), e32))
+    assert rel(Y, Yo) <= max(2e-5, 10 * e32)
     # The GPU's branch: V of every iteration, supports, flips at rounding.
     dev = torch.device('cuda:0')
     V = unrolled_tape(torch.tensor(D, device=dev), torch.tensor(S, device=dev),

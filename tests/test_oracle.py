@@ -407,9 +407,12 @@ def scalar_autorho(sv, lm, rho, alpha, n, period, tau, mu, xi, autoscaling):
 
 AUTORHO_SCALAR = [
     # (s, lambda, rho, alpha, iterations, period, tau, mu, xi, autoscaling):
-    # rho rises, then falls (autoscaled factors); alternates by tau; rises
-    # then falls every iteration (autoscaled).
-    (-0.8, 0.2, 0.1, 1.5, 12, 2, 10.0, 1.2, 2.0, True),
+    # rho rises, then falls (autoscaled factors); falls by tau, then holds;
+    # rises then falls (autoscaled).  Each case stops while the residuals
+    # are far above FP32 rounding (the first one converges fast: after 6
+    # iterations |x - y| is ~3e-5 and the autoscaled factor sqrt(r / (xi s))
+    # would be set by rounding noise in the GPU's FP32 residuals).
+    (-0.8, 0.2, 0.1, 1.5, 6, 2, 10.0, 1.2, 2.0, True),
     (1.0, 0.05, 1.0, 1.8, 12, 1, 3.0, 1.2, 2.0, False),
     (2.0, 0.5, 0.1, 1.5, 12, 1, 3.0, 1.2, 2.0, True),
 ]
