@@ -113,6 +113,32 @@ int scc_cbpdn_grd(const float *D, const float *S, float lambda, float mu,
 int scc_cbpdn_device(const float *D, const float *S, float lambda, float mu,
                      int grad, const scc_algopt *opt, float *Y, void *stream);
 
+/* Bytes of the fixed device array that holds the workspace of
+ * scc_cbpdn_device_noalloc (one copy per GPU, held until the process
+ * ends): 128 MiB. */
+#define SCC_NOALLOC_SCRATCH_BYTES 134217728
+
+/* Allocation-free form of scc_cbpdn_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * that prepares HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments, results and stream semantics as
+ * scc_cbpdn_device, and the same kernels; it calls no hipMalloc, hipFree
+ * or other allocating routine.  The solver state lives in Y, every other
+ * buffer (filter and work spectra, U, the image spectrum and small tables:
+ * about 16 M Hc + 4 M N + 12 Hc bytes, Hc = R (C/2 + 1), N = R C) in a
+ * fixed device array of SCC_NOALLOC_SCRATCH_BYTES per GPU; a problem that
+ * needs more returns SCC_ERR_UNSUPPORTED before any device work (e.g.
+ * 256 x 256 with 32 filters needs 25 MB, 512 x 512 with 32 filters 101 MB,
+ * 512 x 512 with 64 filters does not fit).  Verbose returns
+ * SCC_ERR_UNSUPPORTED (its objective buffers would need allocation);
+ * per-iteration history and stage timing are not recorded.  Every call on
+ * a GPU shares that GPU's array, which keeps the last call's contents:
+ * calls that overlapped on one GPU would race, so the library serialises
+ * them per GPU (each returns after its solve has completed). */
+int scc_cbpdn_device_noalloc(const float *D, const float *S, float lambda,
+                             float mu, int grad, const scc_algopt *opt,
+                             float *Y, void *stream);
+
 /* Info and error text of the last call on this thread (no reference
  * counterpart: the reference prints and exits). */
 int scc_last_info(scc_info *info);

@@ -179,6 +179,7 @@ struct Workspace {
   hipStream_t stream = nullptr;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   Ctrl* hctrl = nullptr;  // host copy of the control block (pinned if owned)
+  Ctrl hctrl_view{};      // its storage in a workspace of views
   std::mutex mu;          // held for the whole of a solve in this workspace
   // Captured iteration loop of the last repeated fixed-iteration solve, and
   // the configuration of the last solve that was not yet captured.
@@ -850,6 +851,131 @@ int solve_host(const float* D, const float* S, float lambda, float mu,
   return SCC_OK;
 }
 
+// --------------------------------------------------------------------------
+// Allocation-free solves (scc_cbpdn_device_noalloc).  Their workspace lives
+// in `scratch`, a file-scope device array: each GPU holds its own copy from
+// the first call until the process ends.  In host code the array's name is
+// only a stand-in for the symbol; the device address of each GPU's copy
+// comes from hipGetSymbolAddress with that GPU current, and only that
+// address is handed to kernels.
+// --------------------------------------------------------------------------
+constexpr size_t kScratchBytes = (size_t)SCC_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices];  // per device, once looked up
+Workspace g_nws[kMaxDevices];               // per device workspace of views
+
+// Device address of this device's scratch array (dev must be current).
+int scratch_base(int dev, unsigned char** base) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr)
+      return fail(SCC_ERR_HIP, "hipGetSymbolAddress(scratch)",
+                  e != hipSuccess ? hipGetErrorString(e) : "null address");
+    g_scratch_ptr[dev] = (unsigned char*)ptr;
+  }
+  *base = g_scratch_ptr[dev];
+  return SCC_OK;
+}
+
+// Lays the buffers of one solve out from base (256-byte aligned each) and
+// returns the bytes they take; base == nullptr only measures.  Y (the
+// solver state and the result) is the caller's output array.
+size_t carve_views(Workspace* ws, unsigned char* base, float* Y) {
+  const Geom& g = ws->g;
+  const size_t nreal = (size_t)g.M * g.N;
+  const size_t ncplx = (size_t)g.M * g.Hc;
+  size_t off = 0;
+  auto take = [&](DevBuf& b, size_t bytes) {
+    b.owned = false;
+    b.p = base ? base + off : nullptr;
+    b.bytes = bytes;
+    off += (bytes + 255) & ~(size_t)255;
+  };
+  auto none = [](DevBuf& b) {
+    b.owned = false;
+    b.p = nullptr;
+    b.bytes = 0;
+  };
+  for (DevBuf* b : {&ws->hist, &ws->sD, &ws->sS, &ws->sL1, &ws->sGW,
+                    &ws->sMask, &ws->oG, &ws->oZ, &ws->oE, &ws->oPart,
+                    &ws->oVal, &ws->oHist})
+    none(*b);
+  ws->Y.owned = false;
+  ws->Y.p = Y;
+  ws->Y.bytes = nreal * 4;
+  take(ws->F, ncplx * 8);
+  take(ws->Df, ncplx * 8);
+  take(ws->U, nreal * 4);
+  take(ws->Sf, (size_t)g.Hc * 8);
+  take(ws->ghg, (size_t)g.Hc * 4);
+  take(ws->twR, (size_t)g.R * 8);
+  take(ws->twC, (size_t)g.C * 8);
+  take(ws->partials, (size_t)5 * ws_row_blocks(ws) * 4);
+  take(ws->blocksum, (size_t)5 * 64 * 8);
+  take(ws->ctrl, sizeof(Ctrl));
+  take(ws->flag, 4);
+  return off;
+}
+
+int solve_noalloc(const float* D, const float* S, float lambda, float mu,
+                  int grad, const scc_algopt* opt, float* Y, void* stream) {
+  if (D == nullptr || S == nullptr || Y == nullptr)
+    return fail(SCC_ERR_INVALID, "NULL array argument");
+  Resolved rs;
+  int rc = resolve(opt, lambda, grad, &rs);
+  if (rc) return rc;
+  const scc_algopt& o = rs.o;
+  if (o.Verbose == 1)
+    return fail(SCC_ERR_UNSUPPORTED,
+                "Verbose needs per-iteration objective buffers, which the "
+                "allocation-free solve does not have");
+  {
+    // Host-side size check first: no device work for a problem that does
+    // not fit the array.
+    Workspace probe;
+    probe.owned = false;
+    if ((rc = init_geometry(&probe, o.IMG_ROW_SIZE, o.IMG_COL_SIZE,
+                            o.DICT_M_SIZE)))
+      return rc;
+    const size_t need = carve_views(&probe, nullptr, Y);
+    if (need > kScratchBytes) {
+      char what[160];
+      std::snprintf(what, sizeof(what),
+                    "allocation-free solve needs %zu bytes of workspace; the "
+                    "fixed scratch array holds %zu",
+                    need, kScratchBytes);
+      return fail(SCC_ERR_UNSUPPORTED, what);
+    }
+  }
+  if ((rc = check_device(o.device))) return rc;
+  if (o.device >= kMaxDevices)
+    return fail(SCC_ERR_UNSUPPORTED, "device index beyond the scratch table");
+  Workspace* ws = &g_nws[o.device];
+  std::lock_guard<std::mutex> lk(ws->mu);
+  ws->owned = false;
+  ws->device = o.device;
+  ws->ghg_ready = false;
+  ws->stream = nullptr;
+  ws->hctrl = &ws->hctrl_view;
+  if ((rc = init_geometry(ws, o.IMG_ROW_SIZE, o.IMG_COL_SIZE, o.DICT_M_SIZE)))
+    return rc;
+  DeviceGuard dg(o.device);
+  unsigned char* base = nullptr;
+  if ((rc = scratch_base(o.device, &base))) return rc;
+  carve_views(ws, base, Y);
+  hipStream_t st = (hipStream_t)stream;
+  // The tables of the previous call's shape may still be in the array.
+  SCC_HIP(launch_twiddles((float2*)ws->twR.p, ws->g.R, st));
+  SCC_HIP(launch_twiddles((float2*)ws->twC.p, ws->g.C, st));
+  return solve_core(ws, rs, D, S, lambda, mu, grad, o.L1Weight, o.GrdWeight,
+                    o.Weight, nullptr, st);
+}
+
 }  // namespace
 }  // namespace scc
 
@@ -938,6 +1064,12 @@ int scc_cbpdn_device(const float* D, const float* S, float lambda, float mu,
   hipStream_t st = stream ? (hipStream_t)stream : ws->stream;
   return solve_core(ws, rs, D, S, lambda, mu, grad, o.L1Weight, o.GrdWeight,
                     o.Weight, Y, st);
+}
+
+int scc_cbpdn_device_noalloc(const float* D, const float* S, float lambda,
+                             float mu, int grad, const scc_algopt* opt,
+                             float* Y, void* stream) {
+  return solve_noalloc(D, S, lambda, mu, grad, opt, Y, stream);
 }
 
 int scc_last_info(scc_info* info) {

@@ -86,6 +86,7 @@ def lib():
         'scc_cbpdn': (_i, [vp, vp, _f, P, vp]),
         'scc_cbpdn_grd': (_i, [vp, vp, _f, _f, P, vp]),
         'scc_cbpdn_device': (_i, [vp, vp, _f, _f, _i, P, vp, vp]),
+        'scc_cbpdn_device_noalloc': (_i, [vp, vp, _f, _f, _i, P, vp, vp]),
         'scc_last_info': (_i, [ctypes.POINTER(Info)]),
         'scc_last_error': (ctypes.c_char_p, []),
         'scc_set_history': (None, [_i]),
