@@ -517,6 +517,10 @@ __device__ inline void row_body_wave(const Geom& g, const FftPlan& rp,
   const float lr = p.lambda / rho;
   const int tsh = __ffs(g.TC) - 1;
   const int C = g.C;
+  // The common case (see the update loop below).
+  const bool plain = g.even && p.vstate && p.m0 == 0 && p.l1_mode != 2 &&
+                     !p.nonneg && !p.nobndry;
+  const bool alpha1 = p.alpha == 1.f;
   UpdateSums acc;
   for (int s = 0; s < nr / kRowWaves; ++s) {
     const long long row = wave_row(g, blk, wv, s, nr, nrows);
@@ -528,16 +532,56 @@ __device__ inline void row_body_wave(const Geom& g, const FftPlan& rp,
     float2* P = fft_wave<-1>(A, B, rp, twL, lane);
     float2* Q = (P == A) ? B : A;
     const long long gi0 = row * C;
-    for (int n = lane; n < C; n += 64) {
-      float* slot = g.even ? &real_at(P, n) : &P[gen::gpad(n)].x;
-      const float a = update_point(g, p, b, *slot * invN, lr, m, r, n,
-                                   gi0 + n, acc,
-                                   p.vstate && mode == kRowFused);
-      if (mode == kRowFused) {
-        if (g.even)
-          *slot = a;
-        else
-          P[gen::gpad(n)] = make_float2(a, 0.f);
+    if (plain) {
+      // Even width, V state, one threshold per row: the real pair
+      // (2k, 2k + 1) is complex slot k, V is read and written as float2,
+      // no per-element option tests (update_point's arithmetic).
+      const float th = lr * (p.l1_mode == 0 ? p.l1_scalar : b.l1w[m]);
+      float2* V2 = reinterpret_cast<float2*>(b.Y + gi0);
+      float2* U2 = reinterpret_cast<float2*>(b.U + gi0);
+      for (int k = lane; k < g.Lr; k += 64) {
+        const float2 xz = P[gen::gpad(k)];
+        const float2 vo = V2[k];
+        const float x0 = xz.x * invN, x1 = xz.y * invN;
+        const float yo0 = copysignf(fmaxf(fabsf(vo.x) - th, 0.f), vo.x);
+        const float yo1 = copysignf(fmaxf(fabsf(vo.y) - th, 0.f), vo.y);
+        const float uo0 = vo.x - yo0, uo1 = vo.y - yo1;
+        const float xr0 = alpha1 ? x0 : p.alpha * x0 + (1.f - p.alpha) * yo0;
+        const float xr1 = alpha1 ? x1 : p.alpha * x1 + (1.f - p.alpha) * yo1;
+        const float v0 = xr0 + uo0, v1 = xr1 + uo1;
+        const float yn0 = copysignf(fmaxf(fabsf(v0) - th, 0.f), v0);
+        const float yn1 = copysignf(fmaxf(fabsf(v1) - th, 0.f), v1);
+        const float un0 = v0 - yn0, un1 = v1 - yn1;
+        acc.x += x0 * x0;
+        acc.y += yn0 * yn0;
+        acc.u += un0 * un0;
+        acc.r += (x0 - yn0) * (x0 - yn0);
+        acc.s += (yo0 - yn0) * (yo0 - yn0);
+        acc.x += x1 * x1;
+        acc.y += yn1 * yn1;
+        acc.u += un1 * un1;
+        acc.r += (x1 - yn1) * (x1 - yn1);
+        acc.s += (yo1 - yn1) * (yo1 - yn1);
+        if (mode == kRowFused) {
+          V2[k] = make_float2(v0, v1);
+          P[gen::gpad(k)] = make_float2(yn0 - un0, yn1 - un1);
+        } else {
+          V2[k] = make_float2(yn0, yn1);
+          U2[k] = make_float2(un0, un1);
+        }
+      }
+    } else {
+      for (int n = lane; n < C; n += 64) {
+        float* slot = g.even ? &real_at(P, n) : &P[gen::gpad(n)].x;
+        const float a = update_point(g, p, b, *slot * invN, lr, m, r, n,
+                                     gi0 + n, acc,
+                                     p.vstate && mode == kRowFused);
+        if (mode == kRowFused) {
+          if (g.even)
+            *slot = a;
+          else
+            P[gen::gpad(n)] = make_float2(a, 0.f);
+        }
       }
     }
     if (mode == kRowFused) {
