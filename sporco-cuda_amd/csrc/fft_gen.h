@@ -228,10 +228,30 @@ __host__ __device__ __forceinline__ GStage make_stage(int L, int Rx, int Ns) {
   return s;
 }
 
+// Where a stage reads and writes: element i of transform t at
+// A + t * lda + i * esa (and B + t * ldb + i * esb for the outputs).  The
+// LDS buffers have (ld, 1); a column panel in global memory (w columns of R
+// rows, row-major) has (1, w).  tsh >= 0: work items take the transforms
+// fastest (t = idx & (nt - 1), nt = 1 << tsh), so that consecutive threads
+// touch adjacent columns of the panel; tsh < 0: the butterflies fastest.
+struct GIo {
+  int lda, esa, ldb, esb, tsh;
+};
+
+__host__ __device__ __forceinline__ GIo gio_lds(int ld) {
+  GIo io;
+  io.lda = ld;
+  io.esa = 1;
+  io.ldb = ld;
+  io.esb = 1;
+  io.tsh = -1;
+  return io;
+}
+
 // Register-butterfly stage (RX <= 14).
 template <int RX, int DIR>
 __host__ __device__ __forceinline__ void stage_reg(const float2* A, float2* B,
-                                                   int ld, int nt,
+                                                   const GIo& io, int nt,
                                                    const GStage& s,
                                                    const float2* tw, int tws,
                                                    int tid, int nthr) {
@@ -241,16 +261,19 @@ __host__ __device__ __forceinline__ void stage_reg(const float2* A, float2* B,
 #pragma unroll 1
   for (int idx = tid; idx < total; idx += nthr) {
     int t = 0, j = idx;
-    if (nt > 1) {
+    if (io.tsh >= 0) {
+      t = idx & (nt - 1);
+      j = idx >> io.tsh;
+    } else if (nt > 1) {
       t = dnb.div(idx);
       j = idx - t * nb;
     }
     const int k = s.p2 ? (j & (Ns - 1)) : j - dns.div(j) * Ns;
-    const float2* in = A + t * ld;
-    float2* out = B + t * ld;
+    const float2* in = A + t * io.lda;
+    float2* out = B + t * io.ldb;
     float2 x[RX];
 #pragma unroll
-    for (int r = 0; r < RX; ++r) x[r] = in[gpad(j + r * nb)];
+    for (int r = 0; r < RX; ++r) x[r] = in[gpad(j + r * nb) * io.esa];
     if (k) {
       // W_L^{e1 r} = (W_L^{e1})^r: one table read, the powers by products
       // of lower powers (at most log2(RX) + 1 products deep).
@@ -264,7 +287,7 @@ __host__ __device__ __forceinline__ void stage_reg(const float2* A, float2* B,
     bfly<RX, DIR>(x);
     const int ob = (j - k) * RX + k;
 #pragma unroll
-    for (int q = 0; q < RX; ++q) out[gpad(ob + q * Ns)] = x[q];
+    for (int q = 0; q < RX; ++q) out[gpad(ob + q * Ns) * io.esb] = x[q];
   }
 }
 
@@ -318,8 +341,8 @@ __host__ __device__ __forceinline__ void stage_prime_twiddle(
 
 template <int DIR>
 __host__ __device__ __forceinline__ void stage_prime(const float2* A,
-                                                     float2* B, int ld, int nt,
-                                                     const GStage& s,
+                                                     float2* B, const GIo& io,
+                                                     int nt, const GStage& s,
                                                      const float2* tw, int tws,
                                                      int tid, int nthr) {
   constexpr int kQB = 4;
@@ -332,7 +355,10 @@ __host__ __device__ __forceinline__ void stage_prime(const float2* A,
 #pragma unroll 1
   for (int idx = tid; idx < total; idx += nthr) {
     int t = 0, rem = idx;
-    if (nt > 1) {
+    if (io.tsh >= 0) {
+      t = idx & (nt - 1);
+      rem = idx >> io.tsh;
+    } else if (nt > 1) {
       t = dper.div(idx);
       rem = idx - t * per;
     }
@@ -340,8 +366,9 @@ __host__ __device__ __forceinline__ void stage_prime(const float2* A,
     const int j = rem - qb * nb;
     const int k = s.p2 ? (j & (Ns - 1)) : j - dns.div(j) * Ns;
     const int q0 = 1 + qb * kQB;
-    const float2* in = A + t * ld;
-    const float2 x0 = in[gpad(j)];
+    const float2* in = A + t * io.lda;
+    const int ea = io.esa;
+    const float2 x0 = in[gpad(j) * ea];
     float2 y0 = x0;
     float2 tq[kQB], vq[kQB], z[kQB], stp[kQB];
     int qq[kQB];
@@ -355,8 +382,8 @@ __host__ __device__ __forceinline__ void stage_prime(const float2* A,
     }
 #pragma unroll 1
     for (int r = 1; r <= h; ++r) {
-      const float2 xa = in[gpad(j + r * nb)];
-      const float2 xb = in[gpad(j + (Rx - r) * nb)];
+      const float2 xa = in[gpad(j + r * nb) * ea];
+      const float2 xb = in[gpad(j + (Rx - r) * nb) * ea];
       const float2 a = c_add(xa, xb);
       const float2 b = c_sub(xa, xb);
       y0 = c_add(y0, a);
@@ -376,15 +403,16 @@ __host__ __device__ __forceinline__ void stage_prime(const float2* A,
         for (int q = 0; q < kQB; ++q) z[q] = c_mul(z[q], stp[q]);
       }
     }
-    float2* out = B + t * ld;
+    float2* out = B + t * io.ldb;
+    const int eb = io.esb;
     const int ob = (j - k) * Rx + k;
-    if (qb == 0) out[gpad(ob)] = y0;
+    if (qb == 0) out[gpad(ob) * eb] = y0;
 #pragma unroll
     for (int q = 0; q < kQB; ++q)
       if (q0 + q <= h) {
         const float2 iv = make_float2(-vq[q].y, vq[q].x);
-        out[gpad(ob + (q0 + q) * Ns)] = c_add(tq[q], iv);
-        out[gpad(ob + (Rx - q0 - q) * Ns)] = c_sub(tq[q], iv);
+        out[gpad(ob + (q0 + q) * Ns) * eb] = c_add(tq[q], iv);
+        out[gpad(ob + (Rx - q0 - q) * Ns) * eb] = c_sub(tq[q], iv);
       }
   }
 }
@@ -452,33 +480,42 @@ __host__ __device__ inline void stage_pre(float2* A, int ld, int nt,
     stage_prime_twiddle<DIR>(A, ld, nt, s, tw, tws, tid, nthr);
 }
 
-// One stage of radix Rx: A -> B.  PFA: also the prime-factor radices 6,
-// 10, 12 and 14 (plans from plan_radices(..., true)).
+// One stage of radix Rx: A -> B with the addressing io.  PFA: also the
+// prime-factor radices 6, 10, 12 and 14 (plans from plan_radices(..., true)).
+template <int DIR, bool PFA = true>
+__host__ __device__ inline void stage_io(const float2* A, float2* B,
+                                         const GIo& io, int nt,
+                                         const GStage& s, const float2* tw,
+                                         int tws, int tid, int nthr) {
+  if constexpr (PFA) {
+    switch (s.Rx) {
+      case 6: stage_reg<6, DIR>(A, B, io, nt, s, tw, tws, tid, nthr); return;
+      case 10: stage_reg<10, DIR>(A, B, io, nt, s, tw, tws, tid, nthr); return;
+      case 12: stage_reg<12, DIR>(A, B, io, nt, s, tw, tws, tid, nthr); return;
+      case 14: stage_reg<14, DIR>(A, B, io, nt, s, tw, tws, tid, nthr); return;
+      default: break;
+    }
+  }
+  switch (s.Rx) {
+    case 2: stage_reg<2, DIR>(A, B, io, nt, s, tw, tws, tid, nthr); break;
+    case 4: stage_reg<4, DIR>(A, B, io, nt, s, tw, tws, tid, nthr); break;
+    case 8: stage_reg<8, DIR>(A, B, io, nt, s, tw, tws, tid, nthr); break;
+    case 3: stage_reg<3, DIR>(A, B, io, nt, s, tw, tws, tid, nthr); break;
+    case 5: stage_reg<5, DIR>(A, B, io, nt, s, tw, tws, tid, nthr); break;
+    case 7: stage_reg<7, DIR>(A, B, io, nt, s, tw, tws, tid, nthr); break;
+    case 11: stage_reg<11, DIR>(A, B, io, nt, s, tw, tws, tid, nthr); break;
+    case 13: stage_reg<13, DIR>(A, B, io, nt, s, tw, tws, tid, nthr); break;
+    default: stage_prime<DIR>(A, B, io, nt, s, tw, tws, tid, nthr); break;
+  }
+}
+
+// One stage between two LDS buffers of stride ld.
 template <int DIR, bool PFA = true>
 __host__ __device__ inline void stage(const float2* A, float2* B, int ld,
                                       int nt, const GStage& s,
                                       const float2* tw, int tws, int tid,
                                       int nthr) {
-  if constexpr (PFA) {
-    switch (s.Rx) {
-      case 6: stage_reg<6, DIR>(A, B, ld, nt, s, tw, tws, tid, nthr); return;
-      case 10: stage_reg<10, DIR>(A, B, ld, nt, s, tw, tws, tid, nthr); return;
-      case 12: stage_reg<12, DIR>(A, B, ld, nt, s, tw, tws, tid, nthr); return;
-      case 14: stage_reg<14, DIR>(A, B, ld, nt, s, tw, tws, tid, nthr); return;
-      default: break;
-    }
-  }
-  switch (s.Rx) {
-    case 2: stage_reg<2, DIR>(A, B, ld, nt, s, tw, tws, tid, nthr); break;
-    case 4: stage_reg<4, DIR>(A, B, ld, nt, s, tw, tws, tid, nthr); break;
-    case 8: stage_reg<8, DIR>(A, B, ld, nt, s, tw, tws, tid, nthr); break;
-    case 3: stage_reg<3, DIR>(A, B, ld, nt, s, tw, tws, tid, nthr); break;
-    case 5: stage_reg<5, DIR>(A, B, ld, nt, s, tw, tws, tid, nthr); break;
-    case 7: stage_reg<7, DIR>(A, B, ld, nt, s, tw, tws, tid, nthr); break;
-    case 11: stage_reg<11, DIR>(A, B, ld, nt, s, tw, tws, tid, nthr); break;
-    case 13: stage_reg<13, DIR>(A, B, ld, nt, s, tw, tws, tid, nthr); break;
-    default: stage_prime<DIR>(A, B, ld, nt, s, tw, tws, tid, nthr); break;
-  }
+  stage_io<DIR, PFA>(A, B, gio_lds(ld), nt, s, tw, tws, tid, nthr);
 }
 
 }  // namespace gen

@@ -306,6 +306,59 @@ __device__ inline void row_fwd_body(const Geom& g, long long nrows,
   r2c_store(g, Z, nt, ld, row0, F, twC, tid, nthr);
 }
 
+// Column transforms of a panel of w columns (w a power of two) straight
+// from and to global memory: the first stage reads its butterflies from the
+// panel (row-major, adjacent columns on adjacent threads) into LDS, the
+// middle stages run between the LDS buffers A and B, and the last writes
+// its outputs back into the panel, which saves the copy-in and copy-out
+// passes through LDS and their barriers.  Needs at least two stages (the
+// panel is read completely, behind a barrier, before it is written).
+template <int DIR>
+__device__ void col_direct(const FftPlan& cp, float2* base, float2* A,
+                           float2* B, int w, int ld,
+                           const float2* __restrict__ tw, int tid, int nthr) {
+  const int nst = uni(cp.nstages), L = uni(cp.L), tws = uni(cp.tw_stride);
+  const int tsh = __ffs(w) - 1;
+  gen::GIo gin;  // panel -> LDS
+  gin.lda = 1;
+  gin.esa = w;
+  gin.ldb = ld;
+  gin.esb = 1;
+  gin.tsh = tsh;
+  gen::GIo gout;  // LDS -> panel
+  gout.lda = ld;
+  gout.esa = 1;
+  gout.ldb = 1;
+  gout.esb = w;
+  gout.tsh = tsh;
+  const gen::GIo gmid = gen::gio_lds(ld);
+  float2* src = A;
+  float2* dst = B;
+  int Ns = 1;
+  for (int s = 0; s < nst; ++s) {
+    const int Rx = uni(cp.radix[s]);
+    const gen::GStage st = gen::make_stage(L, Rx, Ns);
+    if (s == 0) {
+      gen::stage_io<DIR, true>(base, A, gin, w, st, tw, tws, tid, nthr);
+    } else {
+      if (gen::stage_pre_needed(st)) {
+        gen::stage_pre<DIR>(src, ld, w, st, tw, tws, tid, nthr);
+        __syncthreads();
+      }
+      if (s == nst - 1) {
+        gen::stage_io<DIR, true>(src, base, gout, w, st, tw, tws, tid, nthr);
+      } else {
+        gen::stage_io<DIR, true>(src, dst, gmid, w, st, tw, tws, tid, nthr);
+        float2* tmp = src;
+        src = dst;
+        dst = tmp;
+      }
+    }
+    if (s < nst - 1) __syncthreads();
+    Ns *= Rx;
+  }
+}
+
 // Column transforms of panel p of map m, in place.
 __device__ inline void col_body(const Geom& g, const FftPlan& cp, float2* F,
                                 const float2* __restrict__ twR, float dir,
@@ -316,6 +369,13 @@ __device__ inline void col_body(const Geom& g, const FftPlan& cp, float2* F,
   float2* B = lds + w * ld;
   float2* base = F + m * g.Hc + (long long)p * g.TC * g.R;
   const int tid = threadIdx.x, nthr = blockDim.x;
+  if (uni(cp.nstages) >= 2 && (w & (w - 1)) == 0) {
+    if (dir > 0.f)
+      col_direct<1>(cp, base, A, B, w, ld, twR, tid, nthr);
+    else
+      col_direct<-1>(cp, base, A, B, w, ld, twR, tid, nthr);
+    return;
+  }
   const int n = g.R * w;
   const FastDiv dw(w);
   for (int i = tid; i < n; i += nthr) {

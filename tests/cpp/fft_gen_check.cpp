@@ -87,6 +87,92 @@ static int check(int L, const std::vector<int>& radices, int tws, int dir,
   return ok ? 0 : 1;
 }
 
+// The column kernels' direct path (generic_common.h col_direct): w
+// transforms of length L held as a row-major panel (element n of column t
+// at P[n w + t]); the first stage reads the panel, the middle ones run
+// between two LDS-like buffers, the last writes the panel in place.
+static int check_panel(int L, const std::vector<int>& radices, int w,
+                       int dir, int nthr) {
+  const int tws = 1;
+  std::vector<float2> tw(L);
+  for (int t = 0; t < L; ++t) {
+    const double a = -2.0 * M_PI * t / L;
+    tw[t] = make_float2((float)cos(a), (float)sin(a));
+  }
+  const int ld = gld(L);
+  std::vector<float2> P(L * w), A(w * ld, make_float2(1e30f, 1e30f)), B = A;
+  std::vector<double> xr(w * L), xi(w * L);
+  srand(L * 17 + w * 5 + (dir > 0));
+  for (int n = 0; n < L; ++n)
+    for (int t = 0; t < w; ++t) {
+      xr[t * L + n] = rand() / (double)RAND_MAX - 0.5;
+      xi[t * L + n] = rand() / (double)RAND_MAX - 0.5;
+      P[n * w + t] = make_float2((float)xr[t * L + n], (float)xi[t * L + n]);
+    }
+  int tsh = 0;
+  while ((1 << tsh) < w) ++tsh;
+  GIo gin, gout;
+  gin.lda = 1;
+  gin.esa = w;
+  gin.ldb = ld;
+  gin.esb = 1;
+  gin.tsh = tsh;
+  gout.lda = ld;
+  gout.esa = 1;
+  gout.ldb = 1;
+  gout.esb = w;
+  gout.tsh = tsh;
+  const GIo gmid = gio_lds(ld);
+  float2* src = A.data();
+  float2* dst = B.data();
+  int Ns = 1;
+  const int nst = (int)radices.size();
+  for (int s = 0; s < nst; ++s) {
+    const GStage st = make_stage(L, radices[s], Ns);
+    if (s > 0 && stage_pre_needed(st))
+      for (int tid = 0; tid < nthr; ++tid) {
+        if (dir > 0)
+          stage_pre<1>(src, ld, w, st, tw.data(), tws, tid, nthr);
+        else
+          stage_pre<-1>(src, ld, w, st, tw.data(), tws, tid, nthr);
+      }
+    const float2* in = s == 0 ? P.data() : src;
+    float2* out = s == nst - 1 ? P.data() : (s == 0 ? A.data() : dst);
+    const GIo& io = s == 0 ? gin : (s == nst - 1 ? gout : gmid);
+    for (int tid = 0; tid < nthr; ++tid) {
+      if (dir > 0)
+        stage_io<1, true>(in, out, io, w, st, tw.data(), tws, tid, nthr);
+      else
+        stage_io<-1, true>(in, out, io, w, st, tw.data(), tws, tid, nthr);
+    }
+    if (s > 0 && s < nst - 1) {
+      float2* tmp = src;
+      src = dst;
+      dst = tmp;
+    }
+    Ns *= radices[s];
+  }
+  double err = 0, ref = 0;
+  for (int t = 0; t < w; ++t)
+    for (int k = 0; k < L; ++k) {
+      double sr = 0, si = 0;
+      for (int n = 0; n < L; ++n) {
+        const double ang =
+            -2.0 * M_PI * dir * (double)((long long)n * k % L) / L;
+        sr += xr[t * L + n] * cos(ang) - xi[t * L + n] * sin(ang);
+        si += xr[t * L + n] * sin(ang) + xi[t * L + n] * cos(ang);
+      }
+      const float2 g = P[k * w + t];
+      err = fmax(err, fabs(g.x - sr) + fabs(g.y - si));
+      ref = fmax(ref, fabs(sr) + fabs(si));
+    }
+  const double rel = err / ref;
+  const bool ok = rel < 3e-6;
+  printf("panel L=%d w=%d stages=%d dir=%d rel_err=%.3e %s\n", L, w, nst,
+         dir, rel, ok ? "ok" : "FAIL");
+  return ok ? 0 : 1;
+}
+
 static int check_div() {
   int bad = 0;
   for (int d = 1; d <= 4100; d += (d < 64 ? 1 : 13)) {
@@ -165,6 +251,15 @@ int main() {
     const int ns = plan_radices(L, r, 12, true);
     bad += check(L, std::vector<int>(r, r + ns), 1, -1, 1, 64);
     bad += check(L, std::vector<int>(r, r + ns), 1, 1, 2, 256);
+  }
+  // Direct panel path of the column kernels (two or more stages).
+  for (int L : {768, 536, 616, 376, 24, 40, 712, 1000, 150}) {
+    int r[12];
+    const int ns = plan_radices(L, r, 12, true);
+    if (ns < 2) continue;
+    for (int w : {1, 2, 4})
+      for (int dir : {1, -1})
+        bad += check_panel(L, std::vector<int>(r, r + ns), w, dir, 256);
   }
   return bad;
 }
