@@ -78,6 +78,15 @@ CONFIGS = {
 }
 
 
+# Every configuration can also be the main workload (--workload), e.g. for
+# a kernel trace of it.
+WORKLOADS.update({
+    'grad-512x512x64': CONFIGS['cfg1 grad 512x512x64'],
+    'gradmask-512x512x64': CONFIGS['cfg1 grad+mask 512x512x64'],
+    'grad-2048x2048x128': CONFIGS['cfg2 grad 2048x2048x128'],
+})
+
+
 def synth_image(H, W, seed):
     """Sum of 20 Gaussian blobs + 0.1 N(0,1), normalised to [0,1], minus a
     circular 5x5 box mean."""
@@ -131,15 +140,18 @@ def model_bytes_iter(M, R, C, grad=False, mask=False):
     return b
 
 
-def stage_bytes(M, R, C):
-    """Algorithmic bytes per launch of each kernel of the iteration."""
+def stage_bytes(M, R, C, grad=False, mask=False):
+    """Algorithmic bytes per launch of each kernel of the iteration (M maps
+    including a masked solve's impulse map)."""
     N = R * C
     Hc = R * (C // 2 + 1)
     return {
         'col_fwd': 16 * M * Hc,                 # read + write the spectra
-        'solve': 24 * M * Hc + 8 * Hc,          # A, D spectra in, X out, S
+        # A, D spectra in, X out, S (+ the gradient spectrum)
+        'solve': 24 * M * Hc + 8 * Hc + (4 * Hc if grad else 0),
         'col_inv': 16 * M * Hc,
-        'row': 8 * M * N + 16 * M * Hc,         # X in, V in/out, A out
+        # X in, V in/out, A out (+ the mask)
+        'row': 8 * M * N + 16 * M * Hc + (4 * N if mask else 0),
     }
 
 
@@ -529,7 +541,7 @@ def main():
     torch.cuda.synchronize()
     L.scc_set_stage_timing(0)
     st = _abi.last_stage_times()
-    sb = stage_bytes(M, R, C)
+    sb = stage_bytes(pb.Mt, R, C, pb.grad, pb.mask)
     per_kernel = {}
     for k, (ms, n) in st.items():
         if n:
@@ -543,7 +555,10 @@ def main():
                 'unit': 'GB/s', 'frac': ach / HBM_PEAK_GBPS,
                 'traffic': load_traffic(args.workload, dom),
                 'kernel': dom, 'algorithmic_bytes': sb[dom]}
-    mb = model_bytes_iter(M, R, C)
+    mb = model_bytes_iter(M, R, C, pb.grad, pb.mask)
+    variant = ('CBPDN with gradient term (mu=%g)%s' % (
+        wl['mu'], ' and mask' if pb.mask else '')) if pb.grad else \
+        'plain CBPDN'
     out = {
         'metric': METRIC, 'value': value, 'unit': 'ADMM iterations/s',
         'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
@@ -551,11 +566,11 @@ def main():
         'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32',
         'data': 'synthetic (seeded Gaussian-blob image, unit-norm Gaussian '
                 'dictionary)',
-        'config': {'workload': '%s: plain CBPDN, %dx%d image, %d filters '
+        'config': {'workload': '%s: %s, %dx%d image, %d filters '
                                '%dx%d, lambda=%g, rho=%g, RelaxParam=%g, '
                                'AutoRho off, fixed iterations'
-                               % (args.workload, R, C, M, K, K, wl['lmbda'],
-                                  wl['rho'], wl['relax']),
+                               % (args.workload, variant, R, C, M, K, K,
+                                  wl['lmbda'], pb.rho, wl['relax']),
                    'timing': '(median %d-iteration solve - median '
                              '0-iteration solve) over %d repeats'
                              % (args.steps, args.repeats)},
@@ -578,7 +593,7 @@ def main():
     if os.path.exists(acc):
         with open(acc) as f:
             out['accuracy'] = json.load(f)
-    if not args.no_cpu:
+    if not args.no_cpu and 'mu' not in wl:
         print('bench: cpu baseline (%.0f s budget, FP32 then FP64)'
               % args.cpu_seconds, file=sys.stderr, flush=True)
         S_h = synth_image(R, C, 1000 + wl['seed'])
