@@ -339,6 +339,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
 k_row_pow2(Geom g, Buffers b, Params p, int mode) {
   using RC = RowCfg<LR>;
   constexpr int E = RC::E, T = RC::T, RPW = RC::RPW;
+  // The state is prefetched with the spectrum where registers allow it, and
+  // for the plain update on V also with 16 points per lane (two wavefronts
+  // per SIMD either way: 1024-point rows 4 % faster).
+  constexpr bool PF = RC::PREFETCH || (VS && !GENERAL && TAPE == 0);
   constexpr int C = 2 * LR;
   constexpr int LP = reg::lds_len(LR);
   __shared__ float2 lds[4 * RPW * LP];
@@ -374,10 +378,10 @@ k_row_pow2(Geom g, Buffers b, Params p, int mode) {
   // Half spectrum in slot order, column L on lane 0; Y and U prefetched.
   float2 v[E], y2[E], u2[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) v[e] = Fm[spec_off<TC>(t + T * e, r, R)];
+  for (int e = 0; e < E; ++e) v[e] = spec_slot<TC, T, E>(Fm, t, e, r, R);
   float2 xL = make_float2(0.f, 0.f);
   if (t == 0) xL = Fm[(LR / TC) * TC * R + r];
-  if (RC::PREFETCH) {
+  if (PF) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       y2[e] = Y2i[lo + T * e];
@@ -413,7 +417,7 @@ k_row_pow2(Geom g, Buffers b, Params p, int mode) {
                      : (p.l1_mode == 1 && !ams ? b.l1w[m] : 0.f);
   const bool bndry_row = p.nobndry && r > R - p.Kh;
   UpdateSums acc;
-  if (!RC::PREFETCH) {
+  if (!PF) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       y2[e] = Y2i[lo + T * e];
@@ -520,13 +524,12 @@ k_row_pow2(Geom g, Buffers b, Params p, int mode) {
   }
   // Per-wavefront partial sums (no block barrier).
   {
-    const float sv[5] = {acc.x, acc.y, acc.u, acc.r, acc.s};
+    float sv[5] = {acc.x, acc.y, acc.u, acc.r, acc.s};
     const int nwaves = gridDim.x * 4;
+    wave_sum5(sv);
+    if (lane == 0)
 #pragma unroll
-    for (int j = 0; j < 5; ++j) {
-      const float s = wave_sum(sv[j]);
-      if (lane == 0) b.partials[j * nwaves + wave] = s;
-    }
+      for (int j = 0; j < 5; ++j) b.partials[j * nwaves + wave] = sv[j];
   }
   if (mode != kRowFused) return;
   reg::fft_wave<LR, E, 1>(z, t, L, b.twC, 2);

@@ -133,6 +133,20 @@ __device__ __forceinline__ float wave_sum(float v) {
   return v;
 }
 
+// wave_sum of five values at once (the same additions in the same order
+// for each, so the results equal five wave_sum calls): the five lane
+// shuffles of a step are issued together and wait once.
+__device__ __forceinline__ void wave_sum5(float (&v)[5]) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    float s[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) s[j] = __shfl_down(v[j], o, 64);
+#pragma unroll
+    for (int j = 0; j < 5; ++j) v[j] += s[j];
+  }
+}
+
 // Reduce the five sums over the block (fixed order, deterministic) and
 // store them to partials[j * nblocks + bid].  red must hold 5 x nwaves.
 template <int NW>
@@ -141,11 +155,10 @@ __device__ __forceinline__ void block_sums_store(const UpdateSums& acc,
                                                  float* partials, int nblocks,
                                                  int bid, int tid) {
   const int lane = tid & 63, wv = tid >> 6;
-  const float v[5] = {acc.x, acc.y, acc.u, acc.r, acc.s};
-  for (int j = 0; j < 5; ++j) {
-    const float s = wave_sum(v[j]);
-    if (lane == 0) red[j][wv] = s;
-  }
+  float v[5] = {acc.x, acc.y, acc.u, acc.r, acc.s};
+  wave_sum5(v);
+  if (lane == 0)
+    for (int j = 0; j < 5; ++j) red[j][wv] = v[j];
   __syncthreads();
   if (tid < 5) {
     float s = 0.f;

@@ -141,5 +141,7 @@ int main() {
   bad += check<2048, 16, 1>(1);
   bad += check<2048, 16, -1>(1);
   bad += check<1024, 8, -1>(2);
+  bad += check<256, 8, -1>(2);
+  bad += check<128, 8, 1>(2);
   return bad;
 }

@@ -36,6 +36,13 @@ CASES = {
     'M17_2048x256': (2048, 256, 4, 17, 'cbpdn', 0.05, None, 10, {}, None),
     'M3_256x2048_grad': (256, 2048, 4, 3, 'cbpdngrd', 0.05, 0.3, 10, {},
                          None),
+    # Rows of 2048 columns: the plain update spans two wavefronts per row
+    # (fused and AutoRho update-only launches, per-map weights); a mask
+    # takes the general one-wavefront kernel.
+    'autorho_vec_256x2048': (256, 2048, 4, 5, 'cbpdn', 0.05, None, 30,
+                             {'AutoRho': {'Enabled': True, 'Period': 2}},
+                             'vec'),
+    'mask_256x2048': (256, 2048, 4, 3, 'cbpdnmsk', 0.05, None, 12, {}, None),
     'autorho_stop_256': (256, 256, 8, 8, 'cbpdn', 0.05, None, 300,
                          {'AutoRho': {'Enabled': True},
                           'RelStopTol': 2e-3}, None),
