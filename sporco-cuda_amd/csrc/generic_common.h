@@ -19,7 +19,7 @@ struct FastDiv {
   int d;
   float inv;
   __device__ __forceinline__ explicit FastDiv(int dd)
-      : d(dd), inv(1.f / (float)dd) {}
+      : d(dd), inv(gen::quot_recip(dd)) {}
   __device__ __forceinline__ int div(int a) const {
     int q = (int)((float)a * inv);
     int r = a - q * d;

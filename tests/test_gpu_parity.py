@@ -164,18 +164,30 @@ def test_golden(gpu, path):
     print('%s: rel. L2 error %.3e, FP32 oracle %.3e, MSE %.3e'
           % (name, err, e32, mse))
     bar = max(2e-5, 10 * e32)
-    if name.startswith('ref04'):
-        # The reference's own test 04 (an N(0, 1) L1 weight per
-        # coefficient, negative ones included) holds sporco's result to MSE
-        # 1e-6 (test_cbpdn.py:90); the soft threshold of a negative weight
-        # jumps at V = 0, and the GPU's result is checked on the branch of
-        # those jumps it took (fp64_branch).
-        assert mse < 1e-6, (err, mse)
+    l1w = c['opt'].get('L1Weight')
+    if c['api'] in ('cbpdn', 'cbpdngrd') and hasattr(l1w, 'ndim') and \
+            np.any(np.asarray(l1w) < 0):
+        # Negative L1 weights (the reference's tests 02-04 and 07-09,
+        # N(0, 1) weights per map or per coefficient): the soft threshold
+        # of a negative weight jumps by 2 |t| at V = 0, and on these
+        # problems the FP64 loop has a negative-weight coefficient within
+        # FP32 rounding of that jump
+        # (|V| / max |V| = 1.4e-6 at iteration 25 of ref04, 7.0e-7 at
+        # iteration 27 of ref08, 3.0e-6 at iteration 4 of ref09), so an
+        # FP32 computation may take the other side of it depending on its
+        # rounding.  The GPU's result is checked on the branch it took
+        # (fp64_branch: every divergence of its residual history must be
+        # such a sign decision) to the ordinary bar; against the unmodified
+        # FP64 loop it is held to the reference's MSE bar when it took the
+        # same branch (1e-8, or test 04's own 1e-6, test_cbpdn.py:90), and
+        # to test 04's 1e-6 where it took another.
         Xb, flips = fp64_branch(c, hist)
         eb = rel_err(X, Xb)
         print('%s: %d inverted sign decision(s) %s; rel. L2 error on the '
               "GPU's branch %.3e" % (name, len(flips), flips, eb))
         assert eb <= bar, (eb, e32, flips)
+        mse_bar = 1e-6 if (flips or name.startswith('ref04')) else 1e-8
+        assert mse < mse_bar, (err, mse, flips)
         return
     assert err <= bar and mse < 1e-8, (err, e32, mse, info)
 
