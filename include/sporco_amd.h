@@ -172,6 +172,14 @@ int scc_last_objective(float *rows, int cap);
 void scc_set_stage_timing(int enabled);
 int scc_last_stage_times(double *ms_total, int *launches, int n);
 
+/* Row kernel choice of later solves on this thread: 1 (default) the fused
+ * row kernels compiled for fixed row lengths (even widths that are
+ * multiples of 8 up to 1632, plain update: no mask, no full-array weight,
+ * NonNegCoef and NoBndryCross off), 0 the run-time-plan kernel everywhere.
+ * Both compute the same values in the same order; the switch exists to
+ * compare them.  No reference counterpart. */
+void scc_set_row_kernels(int compile_time);
+
 /* Graph replay of repeated fixed-iteration solves on this thread (default
  * on): a solve with no stopping tolerance, no Verbose output, objective or
  * stage timing whose configuration (sizes, parameters, buffers, weight and

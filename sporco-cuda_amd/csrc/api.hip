@@ -158,6 +158,7 @@ struct GraphKey {
   Params p;
   Buffers b;
   int maxit;
+  int row_ct;  // compile-time-length row kernels in use
 };
 
 // Everything that depends only on (device, R, C, M): buffers, plans, tables.
@@ -723,6 +724,7 @@ int solve_core(Workspace* ws, const Resolved& rs, const float* D,
     std::memcpy(&key.p, &p, sizeof(p));
     std::memcpy(&key.b, &b, sizeof(b));
     key.maxit = maxit;
+    key.row_ct = row_ct_enabled() ? 1 : 0;
     if (ws->gexec != nullptr && std::memcmp(&key, &ws->gkey, sizeof(key)) ==
                                     0) {
       SCC_HIP(hipGraphLaunch(ws->gexec, st));
@@ -1099,6 +1101,8 @@ int scc_last_history(float* rsr, int cap) {
 }
 
 void scc_set_stage_timing(int enabled) { g_stage_on = enabled != 0; }
+
+void scc_set_row_kernels(int compile_time) { set_row_ct(compile_time != 0); }
 
 void scc_set_solve_graph(int enabled) { g_graph_on = enabled != 0; }
 

@@ -1138,6 +1138,7 @@ int scc_batched_cbpdn(const float* D, int M, int Kh, int Kw, int nimg,
         key.insert(key.end(), b8, b8 + sizeof(v));
       }
       key.push_back(w->vs ? 1 : 0);
+      key.push_back(row_ct_enabled() ? 1 : 0);
     }
     if (w->exec == nullptr || key != w->gkey) {
       if (w->exec) {

@@ -440,7 +440,7 @@ __host__ __device__ __forceinline__ void stage_prime(const float2* A,
 // LDS and fewer partly idle item rounds (measured: 768-point columns 12 %
 // faster); the row kernels, whose extra stage code cost more than the
 // merge gained, take plans without them.
-inline int plan_radices(int L, int* radix, int max, bool pfa = true) {
+constexpr int plan_radices(int L, int* radix, int max, bool pfa = true) {
   int ns = 0;
   auto push = [&](int r) {
     if (ns >= max) return false;

@@ -319,6 +319,11 @@ hipError_t launch_row(const Geom& g, const FftPlan& rp, const Buffers& b,
   const int nr = row_nr(g);
   const int nb = row_blocks(g);
   const size_t shm = row_lds(g, nr);
+  nblocks_out[0] = nb;
+  if (mode == kRowFused && g.even && p.vstate && p.m0 == 0 &&
+      p.l1_mode != 2 && !p.nonneg && !p.nobndry && row_wave_fits(g) &&
+      row_ct_enabled() && row_ct_supported(g.Lr))
+    return launch_row_ct(g, b, p, nr, nb, shm, st);
   if (row_wave_fits(g))
     hipLaunchKernelGGL(k_row_generic<true>, dim3(nb), dim3(256), shm, st, g,
                        rp, b, p, mode, nr);

@@ -147,6 +147,15 @@ hipError_t launch_row(const Geom& g, const FftPlan& rp, const Buffers& b,
 hipError_t launch_reduce(const Buffers& b, const Params& p, int nblocks,
                          int iter, int max_iter, hipStream_t st);
 int row_blocks(const Geom& g);
+// Fused row kernels with a compile-time row length (rows_ct.hip): whether
+// Lr has one, and its launch (even width, plain update path, wavefront
+// rows; same results as launch_row's kernel).
+bool row_ct_supported(int LR);
+// Whether launch_row uses them (this thread; default on).
+void set_row_ct(bool on);
+bool row_ct_enabled();
+hipError_t launch_row_ct(const Geom& g, const Buffers& b, const Params& p,
+                         int nr, int nb, size_t shm, hipStream_t st);
 // flag[0] = 1 if any of the n device values is negative, else 0.
 hipError_t launch_any_negative(const float* w, long long n, int* flag,
                                hipStream_t st);
