@@ -352,13 +352,14 @@ __host__ __device__ __forceinline__ void stage_prime_twiddle(
   }
 }
 
-template <int DIR>
+// kQB: output pairs per work item (4 with run-time plans; the kernels with
+// compile-time lengths pick it per stage).
+template <int DIR, int kQB = 4>
 __host__ __device__ __forceinline__ void stage_prime(const float2* A,
                                                      float2* B, const GIo& io,
                                                      int nt, const GStage& s,
                                                      const float2* tw, int tws,
                                                      int tid, int nthr) {
-  constexpr int kQB = 4;
   const int Rx = s.Rx, nb = s.nb, Ns = s.Ns;
   const int h = (Rx - 1) / 2;
   const int nq = (h + kQB - 1) / kQB;

@@ -37,6 +37,27 @@ struct RowPlan {
   static constexpr CtPlan p = make();
 };
 
+// Output pairs per work item of a direct-DFT stage of prime P with nb
+// butterflies on one wavefront: the item rounds times the per-input cost
+// (two loads and the pair sums, then 8 instructions per output pair),
+// smallest for at most 4 pairs (more cost registers the kernel does not
+// have).  E.g. 4 x 89, 4 x 67 and 12 x 31 take 3 pairs: one round of items
+// whose inner loop is a quarter shorter than with 4.
+constexpr int ct_prime_pairs(int nb, int P) {
+  const int h = (P - 1) / 2;
+  int best = 4;
+  long long best_cost = -1;
+  for (int q = 1; q <= 4; ++q) {
+    const int items = nb * ((h + q - 1) / q);
+    const long long cost = (long long)((items + 63) / 64) * h * (8 + 8 * q);
+    if (best_cost < 0 || cost < best_cost) {
+      best_cost = cost;
+      best = q;
+    }
+  }
+  return best;
+}
+
 // Forward (DIR = 1) or inverse transform of one row of length L held in
 // LDS at A (ping-pong with B) by one wavefront; returns the buffer with the
 // result.  tw: the row twiddle table W_{2L}^t (stride 2 for W_L).
@@ -54,7 +75,8 @@ __device__ __forceinline__ float2* fft_ct(float2* A, float2* B,
         gen::stage_prime_twiddle<DIR>(A, 0, 1, st, tw, 2, lane, 64);
         reg::wave_sync();
       }
-      gen::stage_prime<DIR>(A, B, gen::gio_lds(0), 1, st, tw, 2, lane, 64);
+      gen::stage_prime<DIR, ct_prime_pairs(L / RX, RX)>(
+          A, B, gen::gio_lds(0), 1, st, tw, 2, lane, 64);
     } else {
       gen::stage_reg<RX, DIR>(A, B, gen::gio_lds(0), 1, st, tw, 2, lane, 64);
     }
