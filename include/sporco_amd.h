@@ -172,13 +172,15 @@ int scc_last_objective(float *rows, int cap);
 void scc_set_stage_timing(int enabled);
 int scc_last_stage_times(double *ms_total, int *launches, int n);
 
-/* Row kernel choice of later solves on this thread: 1 (default) the fused
- * row kernels compiled for fixed row lengths (even widths that are
- * multiples of 8 up to 1632, plain update: no mask, no full-array weight,
- * NonNegCoef and NoBndryCross off), 0 the run-time-plan kernel everywhere.
- * Both compute the same values in the same order; the switch exists to
- * compare them.  No reference counterpart. */
-void scc_set_row_kernels(int compile_time);
+/* Kernel choice of later solves on this thread: 1 (default) the kernels
+ * compiled for fixed transform lengths where one exists (fused row kernels
+ * for even widths that are multiples of 8 up to 1632 with the plain update:
+ * no mask, no full-array weight, NonNegCoef and NoBndryCross off; column
+ * transforms for heights that are multiples of 8 from 32 to 1024), 0 the
+ * run-time-plan kernels everywhere.  Both compute the same expressions in
+ * the same order; the switch exists to compare them.  No reference
+ * counterpart. */
+void scc_set_fixed_kernels(int on);
 
 /* Graph replay of repeated fixed-iteration solves on this thread (default
  * on): a solve with no stopping tolerance, no Verbose output, objective or

@@ -1102,7 +1102,7 @@ int scc_last_history(float* rsr, int cap) {
 
 void scc_set_stage_timing(int enabled) { g_stage_on = enabled != 0; }
 
-void scc_set_row_kernels(int compile_time) { set_row_ct(compile_time != 0); }
+void scc_set_fixed_kernels(int on) { set_row_ct(on != 0); }
 
 void scc_set_solve_graph(int enabled) { g_graph_on = enabled != 0; }
 

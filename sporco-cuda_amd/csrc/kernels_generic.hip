@@ -298,6 +298,9 @@ hipError_t launch_col(const Geom& g, int nmaps, const FftPlan& cp, float2* F,
                       double* bsum) {
   const size_t shm = (size_t)2 * g.TC * gen::gld(g.R) * sizeof(float2);
   const int nred = nparts > 0 ? pow2_defer_blocks(g, nmaps) : 0;
+  if (row_ct_enabled() && col_ct_supported(g.R))
+    return launch_col_ct(g, nmaps, F, twR, inverse, ctrl, parts, nparts, bsum,
+                         nred, shm, st);
   hipLaunchKernelGGL(k_col_generic, dim3(g.P, nmaps), dim3(256), shm, st, g,
                      cp, F, twR, inverse ? -1.f : 1.f, ctrl, parts, nparts,
                      bsum, nred);

@@ -151,7 +151,14 @@ int row_blocks(const Geom& g);
 // Lr has one, and its launch (even width, plain update path, wavefront
 // rows; same results as launch_row's kernel).
 bool row_ct_supported(int LR);
-// Whether launch_row uses them (this thread; default on).
+// Column transforms with a compile-time length (cols_ct.h): whether R has
+// one, and its launch (the arguments of launch_col's kernel).
+bool col_ct_supported(int R);
+hipError_t launch_col_ct(const Geom& g, int nmaps, float2* F,
+                         const float2* twR, int inverse, const Ctrl* ctrl,
+                         const float* parts, int nparts, double* bsum,
+                         int nred, size_t shm, hipStream_t st);
+// Whether launch_row and launch_col use them (this thread; default on).
 void set_row_ct(bool on);
 bool row_ct_enabled();
 hipError_t launch_row_ct(const Geom& g, const Buffers& b, const Params& p,

@@ -121,7 +121,7 @@ def lib():
                                    _i, vp, vp]),
         'scc_unrolled_last_error': (ctypes.c_char_p, []),
         'scc_last_stage_times': (_i, [vp, vp, _i]),
-        'scc_set_row_kernels': (None, [_i]),
+        'scc_set_fixed_kernels': (None, [_i]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
