@@ -106,6 +106,14 @@ k_row_ct(Geom g, Buffers b, Params p, int nr) {
   const int tsh = __ffs(g.TC) - 1;
   const bool alpha1 = p.alpha == 1.f;
   const long long blk = blockIdx.x;
+  // Half-spectrum addressing in 32 bits (panel_off_sh's layout): column
+  // c of row r sits at c + (c >> tsh) K + r TC, K = TC (R - 1), in the
+  // full panels (c < c0 = (P - 1) TC) and at (P - 1) TC R + r wlast + c - c0
+  // in the last one.  Lane 0 takes the Nyquist column L beside column 0,
+  // so the loops run L / 64 rounds.
+  const unsigned K = (unsigned)(g.TC * (g.R - 1));
+  const int c0 = (g.P - 1) * g.TC;
+  const unsigned lastp = (unsigned)(c0 * g.R);
   UpdateSums acc;
   for (int s = 0; s < nr / kRowWaves; ++s) {
     const long long row = wave_row(g, blk, wv, s, nr, nrows);
@@ -115,8 +123,15 @@ k_row_ct(Geom g, Buffers b, Params p, int nr) {
     float2* Fm = b.F + (long long)m * g.Hc;
     // Half spectrum row -> B, inverse real-FFT pre-processing -> A
     // (wave_c2r_load).
+    const unsigned rT = (unsigned)(r * g.TC);
+    const unsigned rL = lastp + (unsigned)(r * g.wlast) - (unsigned)c0;
+    auto at = [&](int c) {
+      return c < c0 ? (unsigned)c + (unsigned)(c >> tsh) * K + rT
+                    : rL + (unsigned)c;
+    };
 #pragma unroll
-    for (int c = lane; c < H; c += 64) B[c] = Fm[panel_off_sh(g, tsh, r, c)];
+    for (int c = lane; c < L; c += 64) B[c] = Fm[at(c)];
+    if (lane == 0) B[L] = Fm[at(L)];
     reg::wave_sync();
 #pragma unroll 1
     for (int k = lane; k < L; k += 64) {
@@ -167,15 +182,16 @@ k_row_ct(Geom g, Buffers b, Params p, int nr) {
     reg::wave_sync();
     const float2* Z = fft_ct<1, L>(P, Q, twL, lane);
     // Forward real-FFT post-processing and store (wave_r2c_store).
-#pragma unroll 1
-    for (int k = lane; k < H; k += 64) {
+    auto post = [&](int k) {
       const float2 zk = Z[k == L ? 0 : k];
       const float2 zn = cconj(Z[k == 0 ? 0 : L - k]);
       const float2 sm = cadd(zk, zn);
       const float2 wd = cmul(twL[k], csub(zk, zn));
-      Fm[panel_off_sh(g, tsh, r, k)] =
-          make_float2(0.5f * (sm.x + wd.y), 0.5f * (sm.y - wd.x));
-    }
+      return make_float2(0.5f * (sm.x + wd.y), 0.5f * (sm.y - wd.x));
+    };
+#pragma unroll 1
+    for (int k = lane; k < L; k += 64) Fm[at(k)] = post(k);
+    if (lane == 0) Fm[at(L)] = post(L);
     reg::wave_sync();  // this row's LDS reads before the next row's writes
   }
   block_sums_store(acc, red, b.partials, gridDim.x, (int)blk, threadIdx.x);
