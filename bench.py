@@ -470,6 +470,56 @@ def load_traffic(workload, stage):
     return d.get(workload, {}).get(stage)
 
 
+def sizes_leg(spec, iters=40, fixed=True):
+    """Per-stage device time of single-image plain solves (64 filters 8x8,
+    lambda 0.02, rho 50, fixed iterations) at the sizes of spec ("RxC,...",
+    or "batch": the 64 sizes of the batched leg), one JSON line per size:
+    average microseconds per launch of the forward column, X-solve, inverse
+    column and row kernels, with their algorithmic bytes per launch.
+    fixed=False runs the run-time-plan kernels only (scc_set_fixed_kernels
+    0)."""
+    import numpy as np
+    import torch
+    from sporco_cuda import _abi
+    if spec == 'batch':
+        rng = np.random.default_rng(3)
+        sizes = [(8 * int(rng.integers(16, 97)), 8 * int(rng.integers(16, 97)))
+                 for _ in range(64)]
+    else:
+        sizes = [tuple(int(v) for v in t.split('x')) for t in spec.split(',')]
+    M, K = 64, 8
+    dev = torch.device('cuda:0')
+    D = torch.from_numpy(synth_dict(K, M, 2003)).to(dev)
+    L = _abi.lib()
+    L.scc_set_fixed_kernels(1 if fixed else 0)
+    w = torch.ones(1, dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    for (R, C) in sizes:
+        S = torch.from_numpy(synth_image(R, C, 1003)).to(dev)
+        Y = torch.empty((M, R, C), dtype=torch.float32, device=dev)
+        o = _abi.make_opt(R, C, M, K, K, 50.0, iters, relax=1.8,
+                          l1_ptr=w.data_ptr())
+        for rep in range(2):
+            L.scc_set_stage_timing(1 if rep == 1 else 0)
+            rc = L.scc_cbpdn_device(D.data_ptr(), S.data_ptr(), 0.02, 0.0, 0,
+                                    ctypes.byref(o), Y.data_ptr(), stream)
+            if rc != 0:
+                raise RuntimeError(_abi.last_error())
+        L.scc_set_stage_timing(0)
+        torch.cuda.synchronize()
+        st = _abi.last_stage_times()
+        Hc = R * (C // 2 + 1)
+        us = {k: 1e3 * v[0] / max(v[1], 1) for k, v in st.items()
+              if k in ('col_fwd', 'solve', 'col_inv', 'row')}
+        print(json.dumps({'size': [R, C], 'us_per_launch': us,
+                          'algorithmic_bytes': {
+                              'col_fwd': 16 * M * Hc, 'col_inv': 16 * M * Hc,
+                              'solve': 24 * M * Hc + 8 * Hc,
+                              'row': 8 * M * R * C + 16 * M * Hc},
+                          'fixed_length_kernels': fixed}), flush=True)
+    L.scc_set_fixed_kernels(1)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -492,6 +542,11 @@ def main():
                     help='only run 4 device copies of 256 MiB (the reference '
                          'kernel that scales the FETCH_SIZE / WRITE_SIZE '
                          'counters to bytes) and exit')
+    ap.add_argument('--sizes', default=None,
+                    help='only time single-image solves of these sizes '
+                         '("RxC,RxC,..." or "batch") per stage and exit')
+    ap.add_argument('--runtime-plans', action='store_true',
+                    help='with --sizes: the run-time-plan kernels only')
     args = ap.parse_args()
     if args.gpus != 1:
         sys.stderr.write('bench.py measures one GPU: this build runs the '
@@ -511,6 +566,9 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({'copy_bytes': a.numel() * 4, 'copies': 4}))
+        return
+    if args.sizes:
+        sizes_leg(args.sizes, fixed=not args.runtime_plans)
         return
     if args.no_plain:
         out = {'batched': None if args.no_batched else
