@@ -603,15 +603,15 @@ __device__ inline void row_body_wave(const Geom& g, const FftPlan& rp,
         const float2 xz = P[gen::gpad(k)];
         const float2 vo = V2[k];
         const float x0 = xz.x * invN, x1 = xz.y * invN;
-        const float yo0 = copysignf(fmaxf(fabsf(vo.x) - th, 0.f), vo.x);
-        const float yo1 = copysignf(fmaxf(fabsf(vo.y) - th, 0.f), vo.y);
-        const float uo0 = vo.x - yo0, uo1 = vo.y - yo1;
+        const float uo0 = split_u(vo.x, th, -th);
+        const float uo1 = split_u(vo.y, th, -th);
+        const float yo0 = vo.x - uo0, yo1 = vo.y - uo1;
         const float xr0 = alpha1 ? x0 : p.alpha * x0 + (1.f - p.alpha) * yo0;
         const float xr1 = alpha1 ? x1 : p.alpha * x1 + (1.f - p.alpha) * yo1;
         const float v0 = xr0 + uo0, v1 = xr1 + uo1;
-        const float yn0 = copysignf(fmaxf(fabsf(v0) - th, 0.f), v0);
-        const float yn1 = copysignf(fmaxf(fabsf(v1) - th, 0.f), v1);
-        const float un0 = v0 - yn0, un1 = v1 - yn1;
+        const float un0 = split_u(v0, th, -th);
+        const float un1 = split_u(v1, th, -th);
+        const float yn0 = v0 - un0, yn1 = v1 - un1;
         acc.x += x0 * x0;
         acc.y += yn0 * yn0;
         acc.u += un0 * un0;

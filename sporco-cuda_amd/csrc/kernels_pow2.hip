@@ -468,7 +468,7 @@ k_row_pow2(Geom g, Buffers b, Params p, int mode) {
   } else {
     // Plain update: Xr = a X + (1 - a) Yold, V = Xr + U,
     // Y = copysign(max(|V| - t, 0), V), U = V - Y.
-    const float th = lr * wmap;
+    const float th = lr * wmap, mth = -th;
     const float a = p.alpha, a1 = 1.f - p.alpha;
     constexpr int TW = TapeWords<E>::W;
     unsigned tb[TW];
@@ -478,20 +478,29 @@ k_row_pow2(Geom g, Buffers b, Params p, int mode) {
     for (int e = 0; e < E; ++e) {
       const float x0 = z[e].x * invN, x1 = z[e].y * invN;
       float yo0 = y2[e].x, yo1 = y2[e].y, uo0, uo1;
-      if (VS) {  // y2 holds V
-        yo0 = copysignf(fmaxf(fabsf(y2[e].x) - th, 0.f), y2[e].x);
-        yo1 = copysignf(fmaxf(fabsf(y2[e].y) - th, 0.f), y2[e].y);
-        uo0 = y2[e].x - yo0;
-        uo1 = y2[e].y - yo1;
+      if (VS) {  // y2 holds V; th >= 0 (split_u)
+        uo0 = split_u(y2[e].x, th, mth);
+        uo1 = split_u(y2[e].y, th, mth);
+        yo0 = y2[e].x - uo0;
+        yo1 = y2[e].y - uo1;
       } else {
         uo0 = u2[e].x;
         uo1 = u2[e].y;
       }
       const float v0 = fmaf(a, x0, fmaf(a1, yo0, uo0));
       const float v1 = fmaf(a, x1, fmaf(a1, yo1, uo1));
-      const float yn0 = copysignf(fmaxf(fabsf(v0) - th, 0.f), v0);
-      const float yn1 = copysignf(fmaxf(fabsf(v1) - th, 0.f), v1);
-      const float un0 = v0 - yn0, un1 = v1 - yn1;
+      float yn0, yn1, un0, un1;
+      if (VS) {
+        un0 = split_u(v0, th, mth);
+        un1 = split_u(v1, th, mth);
+        yn0 = v0 - un0;
+        yn1 = v1 - un1;
+      } else {
+        yn0 = copysignf(fmaxf(fabsf(v0) - th, 0.f), v0);
+        yn1 = copysignf(fmaxf(fabsf(v1) - th, 0.f), v1);
+        un0 = v0 - yn0;
+        un1 = v1 - yn1;
+      }
       // Support |V| > t is Y != 0, and there sign V = sign Y.
       if (TAPE == 1)
         tb[e / 8] |= ((unsigned)(yn0 != 0.f) | ((unsigned)(yn0 < 0.f) << 1) |

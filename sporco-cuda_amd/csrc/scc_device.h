@@ -7,6 +7,17 @@
 
 namespace scc {
 
+// The split of V by a soft threshold t >= 0 into U = V - shrink(V, t) and
+// Y = shrink(V, t) = copysign(max(|V| - t, 0), V): U is V clamped to
+// [-t, t] (one v_med3_f32) and Y = V - U.  Where |V| > t both forms take
+// the same subtraction |V| - t, so they agree bit for bit, except that Y is
+// +0 rather than copysign(0, V) where |V| <= t (no sum, product, stored
+// state or support test tells the two zeros apart).  Two instructions
+// instead of four per value.  Only for t >= 0 (the one-array state).
+__device__ __forceinline__ float split_u(float v, float t, float mt) {
+  return __builtin_amdgcn_fmed3f(v, mt, t);
+}
+
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) {
   return make_float2(a.x + b.x, a.y + b.y);
 }
